@@ -1,0 +1,159 @@
+"""Benchmark of the hourly land-surface step: cell-hours/s on one MI355X.
+
+Workload: 95,333 land cells (0.5-degree global land count) stepped hour by hour, one
+kernel launch per hour, mixed precision (FP32 state and bulk arithmetic, FP64 canopy and
+soil solves), synthetic seeded cells and forcing with the benchmark distributions
+(``ibis_amd.synth``).  A "step" is one hour over all cells; ``--steps`` hours are timed
+after ``--warmup`` untimed hours.  The forcing of every hour is generated before the timed
+region and kept in HBM; each timed hour copies its 12 forcing planes into the state (as the
+reference driver writes ``state.atm`` before each step, ``loops_gswp3.py:142-152``) and
+launches the step.
+
+Reported beside the throughput:
+
+* ``roofline``: the step kernel's achieved algorithmic bandwidth -- 1,448 B per cell-hour
+  (179 fp32 words read, 180 written, 12 int8 slot ids) x cells / mean kernel duration
+  measured with HIP events on the launch stream -- against MI355X's 8 TB/s;
+* ``cpu_baseline``: the oracle (NumPy FP64 restatement of the reference step, one core) on a
+  bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=95333)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "f32", "f64"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-hours", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this build has a single MI355X; cells "
+              "shard across GPUs with no exchange, so N GPUs run N independent shards.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ibis_amd import layout as L, synth
+    from ibis_amd.params import default_sim_params, get_constants
+    from ibis_amd.state import IBISState
+    from ibis_amd import step as S
+    from oracle import params as OP
+
+    n = args.cells
+    hours = args.warmup + args.steps
+    dev = torch.device("cuda:0")
+    dtype = torch.float64 if args.precision == "f64" else torch.float32
+    pitch = (n + 255) // 256 * 256
+    fields, slot, lat = synth.make_cells(n, args.seed)
+    st = IBISState.from_numpy(fields, slot, synth.HSOI, OP.root_fractions(), device=dev,
+                              dtype=dtype, pitch=pitch)
+    consts, params = get_constants(), default_sim_params(bedrock=True)
+    cparams = S.ibis_params(st, consts, params)
+
+    atm0 = L.PLANE_OFFSET["atm.ta"]
+    natm = sum(f.width for f in L.fields_of("atm"))
+    forc = torch.zeros((hours, natm, pitch), dtype=dtype, device=dev)
+    for h in range(hours):
+        fh = synth.forcing(n, args.seed, h, lat, fields["atm.psurf"])
+        host = np.zeros((natm, pitch))
+        for f in L.fields_of("atm"):
+            off = L.PLANE_OFFSET[f.key] - atm0
+            v = np.asarray(fh[f.key])
+            if f.width == 1:
+                host[off, :n] = v
+            else:
+                host[off:off + f.width, :n] = v.T
+        forc[h] = torch.from_numpy(host).to(dev, dtype)
+    atm_view = st.arena[atm0:atm0 + natm]
+    stream = torch.cuda.current_stream(dev)
+
+    def hour(h, ev=None):
+        atm_view.copy_(forc[h])
+        if ev is not None:
+            ev[0].record(stream)
+        S.timestep_(st, consts, params, 3600.0, precision=args.precision, stream=stream,
+                    cparams=cparams)
+        if ev is not None:
+            ev[1].record(stream)
+
+    for h in range(args.warmup):
+        hour(h)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        hour(args.warmup + k, evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in evs]
+    kmean = float(np.mean(kms)) * 1e-3
+    out = st.to_numpy(keys=("soil.tsoi", "canopy.tu"))
+    finite = float(np.mean(np.isfinite(out["soil.tsoi"]).all(1) & np.isfinite(out["canopy.tu"])))
+
+    value = n * args.steps / wall
+    per_cell = L.ALGORITHMIC_BYTES_PER_CELL_HOUR
+    achieved = per_cell * n / kmean
+    res = {
+        "metric": "cell-hours/s", "value": value, "unit": "cell-hours/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"mixed": "f32+f64", "f32": "f32", "f64": "f64"}[args.precision],
+        "data": "synthetic (seeded cells and hourly forcing, benchmark distributions)",
+        "config": {"workload": f"plain hourly step, {n:,} cells x {args.steps} h, one launch per hour",
+                   "cells": n, "hours": args.steps, "precision": args.precision,
+                   "soil": "4 m soil + 8 m bedrock", "plant_type_slots": 12},
+        "kernel_ms_mean": kmean * 1e3, "kernel_ms_min": float(np.min(kms)),
+        "finite_fraction": finite,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_cell_hour": per_cell},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(n, args.cpu_hours, args.seed)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(n, hours, seed):
+    """The oracle (NumPy FP64, one core) on the first ``hours`` hours of the same cells."""
+    from ibis_amd import synth
+    from ibis_amd.params import default_sim_params, get_constants
+    from oracle import params as OP, run as orun
+    fields, slot, lat = synth.make_cells(n, seed)
+    consts, params = get_constants(), default_sim_params(bedrock=True)
+    st = dict(fields)
+    t = 0.0
+    for h in range(hours):
+        st.update(synth.forcing(n, seed, h, lat, fields["atm.psurf"]))
+        t0 = time.perf_counter()
+        out = orun.run_step(st, slot, consts, params, hsoi=synth.HSOI, froot=OP.root_fractions(),
+                            nytimes=float(h))
+        t += time.perf_counter() - t0
+        for k, v in out.items():
+            if not k.startswith("_"):
+                st[k] = v
+    return {"value": n * hours / t, "unit": "cell-hours/s", "cores": 1, "kind": "port",
+            "sample": f"{n:,} cells x {hours} h, NumPy FP64 oracle, one thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,47 @@
+"""Loader of the in-tree HIP library ``libibis_step.so`` (C ABI of ``include/ibis_step.h``).
+
+There is no fallback: if the library is missing or cannot be loaded, the step raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .abi import IbisArena, IbisParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibis_step.so")
+
+#: every entry point declared in include/ibis_step.h
+EXPORTS = ("ibis_step", "ibis_last_error", "ibis_version")
+
+_lib = None
+
+
+class IbisError(RuntimeError):
+    pass
+
+
+def load():
+    """Load and type the library once; raise ImportError with the build hint if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950) first")
+    lib = C.CDLL(LIB_PATH)
+    lib.ibis_step.argtypes = [C.POINTER(IbisArena), C.POINTER(IbisArena), C.c_void_p, C.c_int64,
+                              C.c_void_p, C.POINTER(IbisParams), C.c_double, C.c_double,
+                              C.c_int64, C.c_int, C.c_void_p]
+    lib.ibis_step.restype = C.c_int
+    lib.ibis_last_error.restype = C.c_char_p
+    lib.ibis_version.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().ibis_last_error().decode(errors="replace")
+        raise IbisError(f"{what} failed with code {rc}: {msg}")

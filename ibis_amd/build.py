@@ -1,0 +1,63 @@
+"""In-tree build of the HIP library (gfx950) and the host-side check library.
+
+``python -m ibis_amd.build`` compiles:
+
+* ``ibis_amd/libibis_step.so``       -- the product: HIP kernels + C ABI, ``--offload-arch=gfx950``;
+* ``ibis_amd/libibis_hostcheck.so``  -- the same step source built for the host with g++,
+  loaded only by tests to check the kernel source against the oracle without a GPU.
+
+Both use ``-ffp-contract=off`` so the FP64 build follows the reference's rounding.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+from . import layout
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             f"-I{INC}"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+              "-Wno-unknown-pragmas", f"-I{INC}"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(line for line in r.stderr.splitlines() if "packed-fp32-ops" not in line)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+    return err
+
+
+def check_layout_header():
+    path = os.path.join(INC, "ibis_layout.h")
+    with open(path) as fh:
+        if fh.read() != layout.render_header():
+            raise RuntimeError("include/ibis_layout.h is stale; regenerate with "
+                               "`python -m ibis_amd.layout > include/ibis_layout.h`")
+
+
+def build(verbose: bool = False):
+    check_layout_header()
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    out = os.path.join(HERE, "libibis_step.so")
+    msg = _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "ibis_step.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    hout = os.path.join(HERE, "libibis_hostcheck.so")
+    msg2 = _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "ibis_hostcheck.cpp"), "-o", hout + ".tmp"])
+    os.replace(hout + ".tmp", hout)
+    if verbose:
+        sys.stderr.write(msg + msg2)
+    return out, hout
+
+
+if __name__ == "__main__":
+    for p in build(verbose=True):
+        print(p)

@@ -1,0 +1,65 @@
+// Host build of the per-cell step source (ibis_step_cell.hpp) for CPU-side logic checks.
+//
+// Test infrastructure only: tests load it to check the kernel's source against the oracle
+// on machines without a GPU.  The product path (libibis_step.so, ibis_amd.step) never
+// loads it.  Same template instantiations, same operation order, no FMA contraction.
+#include <math.h>
+#include <stdint.h>
+
+#include "ibis_step_cell.hpp"
+
+namespace {
+
+template <class S, class R, class D>
+void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
+         int64_t slot_pitch, const double* table, const IbisParams& P, const ibis::StepConst& K,
+         int copy_in) {
+  for (int64_t i = 0; i < ncell; ++i) {
+    R vrow[IBIS_NPFT];
+    for (int j = 0; j < IBIS_NPFT; ++j) {
+      const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
+      const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
+      vrow[j] = R(table[row]);
+    }
+    if (copy_in) {
+#define X(F, W, OFF, ROLE)                                                            \
+  if ((ROLE) == IBIS_ROLE_IN)                                                         \
+    for (int j = 0; j < (W); ++j)                                                     \
+      out[(int64_t)((OFF) + j) * pitch + i] = in[(int64_t)((OFF) + j) * pitch + i];
+      IBIS_FIELD_TABLE
+#undef X
+    }
+    ibis::Cell<S> c{in, out, pitch, i};
+    ibis::step_cell<S, R, D>(c, vrow, P, K);
+  }
+}
+
+}  // namespace
+
+extern "C" int ibis_hostcheck_step(const void* in, void* out, int64_t pitch, int64_t ncell,
+                                   const int8_t* slot, int64_t slot_pitch, const double* table,
+                                   const IbisParams* P, double dtime, double nytimes, int flags) {
+  ibis::StepConst K;
+  K.dtime = dtime;
+  K.tfac = 1.0 / pow(0.999, P->cappa);
+  K.zweight = exp(-1.0 / (10.0 * 86400.0 / dtime));
+  K.nytimes_new = nytimes + 1.0;
+  K.bedrock = (flags & IBIS_FLAG_BEDROCK) ? 1 : 0;
+  const int copy_in = (flags & IBIS_FLAG_COPY_IN) ? 1 : 0;
+  switch (flags & IBIS_PREC_MASK) {
+    case IBIS_PREC_MIXED:
+      run<float, float, double>((const float*)in, (float*)out, pitch, ncell, slot, slot_pitch,
+                                table, *P, K, copy_in);
+      return 0;
+    case IBIS_PREC_F32:
+      run<float, float, float>((const float*)in, (float*)out, pitch, ncell, slot, slot_pitch,
+                               table, *P, K, copy_in);
+      return 0;
+    case IBIS_PREC_F64:
+      run<double, double, double>((const double*)in, (double*)out, pitch, ncell, slot,
+                                  slot_pitch, table, *P, K, copy_in);
+      return 0;
+    default:
+      return -2;
+  }
+}

@@ -1,0 +1,103 @@
+// Scalar math for the per-cell step: precision-generic wrappers, NaN-propagating
+// min/max/clip (the jnp.maximum / jnp.minimum / jnp.clip semantics of the reference),
+// integer powers in JAX's square-and-multiply order, and the Lowe (1977) saturation
+// thermodynamics of python-port/ibis/physics/thermo.py:65-224.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define IBIS_HD __host__ __device__
+#define IBIS_INLINE __forceinline__
+#else
+#define IBIS_HD
+#define IBIS_INLINE inline
+#endif
+
+namespace ibis {
+
+IBIS_HD IBIS_INLINE float r_exp(float x) { return expf(x); }
+IBIS_HD IBIS_INLINE double r_exp(double x) { return exp(x); }
+IBIS_HD IBIS_INLINE float r_log(float x) { return logf(x); }
+IBIS_HD IBIS_INLINE double r_log(double x) { return log(x); }
+IBIS_HD IBIS_INLINE float r_sqrt(float x) { return sqrtf(x); }
+IBIS_HD IBIS_INLINE double r_sqrt(double x) { return sqrt(x); }
+IBIS_HD IBIS_INLINE float r_pow(float x, float y) { return powf(x, y); }
+IBIS_HD IBIS_INLINE double r_pow(double x, double y) { return pow(x, y); }
+IBIS_HD IBIS_INLINE float r_abs(float x) { return fabsf(x); }
+IBIS_HD IBIS_INLINE double r_abs(double x) { return fabs(x); }
+IBIS_HD IBIS_INLINE float r_floor(float x) { return floorf(x); }
+IBIS_HD IBIS_INLINE double r_floor(double x) { return floor(x); }
+
+// jnp.maximum / jnp.minimum: a NaN in either operand gives NaN.
+template <class R> IBIS_HD IBIS_INLINE R mx(R a, R b) { return (a > b || a != a) ? a : b; }
+template <class R> IBIS_HD IBIS_INLINE R mn(R a, R b) { return (a < b || a != a) ? a : b; }
+template <class R> IBIS_HD IBIS_INLINE R clip(R x, R lo, R hi) { return mn(mx(x, lo), hi); }
+
+// x**n in the multiplication order of lax.integer_pow / jnp.power with an integer
+// exponent: the powers x^(2^i) of the set bits of n are multiplied in increasing i.
+template <class R> IBIS_HD IBIS_INLINE R ipow(R x, int n) {
+  R acc = (x == R(0) && n != 0) ? R(0) : R(1);
+  R base = x;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    if (n & 1) acc = acc * base;
+    base = base * base;
+    n >>= 1;
+  }
+  return acc;
+}
+template <class R> IBIS_HD IBIS_INLINE R sq(R x) { return x * x; }
+template <class R> IBIS_HD IBIS_INLINE R cube(R x) { return x * (x * x); }
+template <class R> IBIS_HD IBIS_INLINE R pow4(R x) { return (x * x) * (x * x); }
+
+// ---- saturation thermodynamics (thermo.py) -----------------------------------------
+template <class R> IBIS_HD IBIS_INLINE R esat(R t) {
+  const R tl = clip(t - R(273.16), R(0), R(100));
+  const R ti = clip(t - R(273.16), R(-60), R(0));
+  const R a0 = (t >= R(273.16)) ? R(6.1078000) : R(6.1091780);
+  R pa = R(6.1368209e-11);
+  pa = R(2.0340809e-8) + tl * pa;
+  pa = R(3.0312404e-6) + tl * pa;
+  pa = R(2.6506485e-4) + tl * pa;
+  pa = R(1.4289458e-2) + tl * pa;
+  pa = R(4.4365185e-1) + tl * pa;
+  R pb = R(1.8388269e-10);
+  pb = R(4.8388032e-8) + ti * pb;
+  pb = R(5.8247203e-6) + ti * pb;
+  pb = R(4.1762237e-4) + ti * pb;
+  pb = R(1.8860134e-2) + ti * pb;
+  pb = R(5.0346990e-1) + ti * pb;
+  return R(100) * (a0 + tl * pa + ti * pb);
+}
+
+template <class R> IBIS_HD IBIS_INLINE R desat(R t) {
+  const R tl = clip(t - R(273.16), R(0), R(100));
+  const R ti = clip(t - R(273.16), R(-60), R(0));
+  const R c0 = (t >= R(273.16)) ? R(4.4381000e-1) : R(5.0303052e-1);
+  R pa = R(-7.0902448e-13);
+  pa = R(3.5324218e-10) + tl * pa;
+  pa = R(1.0365614e-7) + tl * pa;
+  pa = R(1.2152151e-5) + tl * pa;
+  pa = R(7.9380540e-4) + tl * pa;
+  pa = R(2.8570026e-2) + tl * pa;
+  R pb = R(7.1310977e-12);
+  pb = R(2.1585425e-9) + ti * pb;
+  pb = R(3.0056931e-7) + ti * pb;
+  pb = R(2.4775631e-5) + ti * pb;
+  pb = R(1.2679954e-3) + ti * pb;
+  pb = R(3.7732550e-2) + ti * pb;
+  return R(100) * (c0 + tl * pa + ti * pb);
+}
+
+template <class R> IBIS_HD IBIS_INLINE R qsat(R e, R p) {
+  return R(0.622) * e / mx(p - R(1.0 - 0.622) * e, R(0.622) * e);
+}
+
+template <class R> IBIS_HD IBIS_INLINE R dqsat(R t, R q1, R e) {
+  return desat(t) * q1 * (R(1) + q1 * R(1.0 / 0.622 - 1.0)) / e;
+}
+
+}  // namespace ibis

@@ -1,0 +1,634 @@
+// Per-cell hourly physics, part A: two-stream radiation, interception cascade, aerodynamics.
+//
+// One thread owns one cell; every quantity here is a scalar in registers.  Each function
+// follows the reference routine named in its comment, in the reference's operation order
+// (the FP64 build reproduces the reference's FP64 arithmetic to rounding).  R is the bulk
+// arithmetic type.
+#pragma once
+
+#include "ibis_math.hpp"
+#include "../../include/ibis_step.h"
+
+namespace ibis {
+
+// ---------------------------------------------------------------------------------------
+// Radiation (python-port/ibis/physics/radiation.py)
+// ---------------------------------------------------------------------------------------
+template <class R> struct TwoSet { R omega, betad, betai, avmu, gdir; };
+
+// twoset (radiation.py:75-159); `upper` selects the leaf/stem weighted snow adjustment.
+template <class R>
+IBIS_HD IBIS_INLINE TwoSet<R> twoset(R coszen, R lai, R sai, R fwl, R rll, R tleaf, R rho, R tau,
+                                     R oriev, R orieh, R omegasno, R tmelt, bool upper, bool sunlit,
+                                     R fws, R rls, R tstem) {
+  TwoSet<R> o;
+  const R orand = R(1) - oriev - orieh;
+  const R omega0 = rho + tau;
+  const R ztab = R(0.5) - coszen / R(3);
+  const R rw = R(1) / mx(omega0, R(1e-12));
+  R betad = (oriev * R(0.5) * (rho + tau) + orieh * rho +
+             orand * ((R(1) - ztab) * rho + ztab * tau)) * rw;
+  R betai = (oriev * R(0.5) * (rho + tau) + orieh * rho +
+             orand * (R(2.0 / 3.0) * rho + R(1.0 / 3.0) * tau)) * rw;
+  R gdir = oriev * R(2.0 / 3.1415927) * r_sqrt(mx(R(0), R(1) - coszen * coszen)) +
+           orieh * coszen + orand * R(0.5);
+  R y, ov;
+  if (upper) {
+    const R x = lai / mx(lai + sai, R(1e-7));
+    y = x * (fwl * (R(1) - rll)) + (R(1) - x) * (fws * (R(1) - rls));
+    const R sl = clip((tmelt - tleaf) / R(0.1), R(0.6), R(1));
+    const R ss = clip((tmelt - tstem) / R(0.1), R(0.6), R(1));
+    ov = (x * sl + (R(1) - x) * ss) * omegasno;
+  } else {
+    y = fwl * (R(1) - rll);
+    ov = omegasno * (R(0.6) + R(0.4) * clip((tmelt - tleaf) / R(1), R(0), R(1)));
+  }
+  const R sno = y * ov;
+  R omega = mx((R(1) - y) * omega0 + sno, R(1e-12));
+  betad = ((R(1) - y) * omega0 * betad + sno * R(0.5)) / omega;
+  betai = ((R(1) - y) * omega0 * betai + sno * R(0.5)) / omega;
+  o.omega = sunlit ? omega : R(0);
+  o.betad = sunlit ? betad : R(0);
+  o.betai = sunlit ? betai : R(0);
+  o.gdir = sunlit ? gdir : R(0);
+  o.avmu = R(1);
+  return o;
+}
+
+template <class R> struct TwoStr {
+  R abvegd, abvegi, refld, refli, fbeldd, fbelid, fbelii, omega;
+  R term[7];
+};
+
+// twostr (radiation.py:180-309)
+template <class R>
+IBIS_HD IBIS_INLINE TwoStr<R> twostr(R coszen, R lai, R sai, R asurd, R asuri, const TwoSet<R>& s,
+                                     R eps, bool sunlit, bool vis) {
+  TwoStr<R> o;
+  const R omega = s.omega, betad = s.betad, betai = s.betai, avmu = s.avmu;
+  const R b = R(1) - omega * (R(1) - betai);
+  const R c = omega * betai;
+  const R tmp0 = b * b - c * c;
+  const R q = r_sqrt(mx(R(0), tmp0));
+  const R k = s.gdir / mx(coszen, R(0.01));
+  R p = avmu * k;
+  const R sgn = (p >= q) ? R(1) : R(-1);
+  if (r_abs(p - q) < R(0.001) * p) p = (R(1) + sgn * R(0.001)) * p;
+  const R c0 = omega * p;
+  const R d = c0 * betad;
+  const R f = c0 * (R(1) - betad);
+  const R h = q / avmu;
+  R sigma = p * p - tmp0;
+  const R ud1 = b - c / mx(asurd, R(1e-12));
+  const R ui1 = b - c / mx(asuri, R(1e-12));
+  const R ud2 = b - c * asurd;
+  const R ui2 = b - c * asuri;
+  const R ud3 = f + c * asurd;
+  const R xai = mx(lai + sai, eps);
+  const R s1 = r_exp(-h * xai);
+  const R s2 = r_exp(-k * xai);
+  const R p1 = b + q, p2 = b - q, p3 = b + p, p4 = b - p;
+  const R s1s = mx(s1, R(1e-30));
+  const R rw = R(1) / s1s;
+  R dd1 = p1 * (ud1 - q) * rw - p2 * (ud1 + q) * s1;
+  R di1 = p1 * (ui1 - q) * rw - p2 * (ui1 + q) * s1;
+  R dd2 = (ud2 + q) * rw - (ud2 - q) * s1;
+  R di2 = (ui2 + q) * rw - (ui2 - q) * s1;
+  if (r_abs(dd1) < R(1e-30)) dd1 = R(1e-30);
+  if (r_abs(di1) < R(1e-30)) di1 = R(1e-30);
+  if (r_abs(dd2) < R(1e-30)) dd2 = R(1e-30);
+  if (r_abs(di2) < R(1e-30)) di2 = R(1e-30);
+  if (r_abs(sigma) < R(1e-30)) sigma = R(1e-30);
+  const R h1 = -d * p4 - c * f;
+  const R rw2 = s2 * (d - c - h1 * (ud1 + p) / sigma);
+  const R h2 = (R(1) / dd1) * ((d - h1 * p3 / sigma) * (ud1 - q) / s1s - p2 * rw2);
+  const R h3 = -(R(1) / dd1) * ((d - h1 * p3 / sigma) * (ud1 + q) * s1 - p1 * rw2);
+  const R h4 = -f * p3 - c * d;
+  const R rw3 = s2 * (ud3 - h4 * (ud2 - p) / sigma);
+  const R h5 = -(R(1) / dd2) * (h4 * (ud2 + q) / (sigma * s1s) + rw3);
+  const R h6 = (R(1) / dd2) * (h4 * s1 * (ud2 - q) / sigma + rw3);
+  const R h7 = c * (ui1 - q) / (di1 * s1s);
+  const R h8 = -c * s1 * (ui1 + q) / di1;
+  const R h9 = (ui2 + q) / (di2 * s1s);
+  const R h10 = -s1 * (ui2 - q) / di2;
+  const R fbeldd = s2;
+  const R fbelid = h4 / sigma * s2 + h5 * s1 + h6 / s1s;
+  const R fbelii = h9 * s1 + h10 / s1s;
+  const R refld = h1 / sigma + h2 + h3;
+  const R refli = h7 + h8;
+  const R absurd = (R(1) - asurd) * fbeldd + (R(1) - asuri) * fbelid;
+  const R absuri = (R(1) - asuri) * fbelii;
+  R abvegd = mx(R(0), R(1) - refld - absurd);
+  R abvegi = mx(R(0), R(1) - refli - absuri);
+  if (xai < eps) { abvegd = R(0); abvegi = R(0); }
+  if (vis) {
+    o.term[0] = k * (R(1) + (h4 - h1) / sigma);
+    o.term[1] = h * (h5 - h2);
+    o.term[2] = h * (h9 - h7);
+    o.term[3] = h * (h3 - h6);
+    o.term[4] = h * (h8 - h10);
+    o.term[5] = k;
+    o.term[6] = h;
+  }
+  o.abvegd = sunlit ? abvegd : R(0);
+  o.abvegi = sunlit ? abvegi : R(0);
+  o.refld = sunlit ? refld : R(0);
+  o.refli = sunlit ? refli : R(0);
+  o.fbeldd = sunlit ? fbeldd : R(0);
+  o.fbelid = sunlit ? fbelid : R(0);
+  o.fbelii = sunlit ? fbelii : R(0);
+  o.omega = omega;
+  return o;
+}
+
+template <class R> struct RadOut {
+  R solu, sols, soll, solg, soli;
+  R asurd[2], asuri[2];
+  R topparu, topparl, scalcoefu[4], scalcoefl[4];
+  R termu6, termu7, terml6, terml7;
+};
+
+template <class R> struct Fwet { R fwetu, fwets, fwetl, rliqu, rliqs, rliql; };
+
+// compute_radiation, solar part: solsur + solalb + solarf per band (radiation.py:316-536,
+// 654-761); the infrared part is irrad below
+template <class R>
+IBIS_HD void radiation(RadOut<R>& o, R coszen, const R solad[2], const R solai[2],
+                       const R lai[2], const R sai[2], R fu, R fl, R tu, R ts, R tl,
+                       R fi, R tsno1, R wsoi1, R wisoi1, R albsav, R albsan, const Fwet<R>& fw,
+                       const IbisParams& P) {
+  const R eps = R(P.epsilon), tmelt = R(P.tmelt);
+  const bool sunlit = coszen > R(0);
+  o.solu = o.sols = o.soll = o.solg = o.soli = R(0);
+  o.topparu = o.topparl = R(0);
+  for (int j = 0; j < 4; ++j) { o.scalcoefu[j] = R(0); o.scalcoefl[j] = R(0); }
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib) {
+    // solsur (radiation.py:316-365)
+    const R zw = wsoi1 * (R(1) - wisoi1);
+    const R dinc = R(1) + clip(R(1) - zw / R(0.5), R(0), R(1));
+    const R base = ib == 0 ? albsav : albsan;
+    R albsod = mn(base * dinc, R(0.8));
+    const R a_hi = ib == 0 ? R(0.70) : R(0.40);
+    const R a_lo = ib == 0 ? R(0.90) : R(0.60);
+    const R t_shi = tmelt, t_slo = tmelt - R(15);
+    R x = (a_hi * (tsno1 - t_slo) + a_lo * (t_shi - tsno1)) / (t_shi - t_slo);
+    x = clip(x, a_hi, a_lo);
+    const R zfac = mx(R(0), R(1.5) / (R(1) + R(4) * coszen) - R(0.5));
+    R albsnd = mn(R(0.99), x + (R(1) - x) * zfac);
+    R albsni = mn(R(1), x);
+    if (!sunlit) { albsod = R(0); albsnd = R(0); albsni = R(0); }
+    const R albsoi = albsod;
+    const R omegasno = ib == 0 ? R(0.9) : R(0.7);
+    // solalb (radiation.py:372-443)
+    const TwoSet<R> sl = twoset(coszen, lai[0], sai[0], fw.fwetl, fw.rliql, tl,
+                                R(P.rhoveg[ib][0]), R(P.tauveg[ib][0]), R(P.oriev[0]),
+                                R(P.orieh[0]), omegasno, tmelt, false, sunlit, R(0), R(0), R(0));
+    const TwoStr<R> lo = twostr(coszen, lai[0], sai[0], albsod, albsoi, sl, eps, sunlit, ib == 0);
+    const R asd = fl * (R(1) - fi) * lo.refld + (R(1) - fl) * (R(1) - fi) * albsod + fi * albsnd;
+    const R asi = fl * (R(1) - fi) * lo.refli + (R(1) - fl) * (R(1) - fi) * albsoi + fi * albsni;
+    const TwoSet<R> su = twoset(coszen, lai[1], sai[1], fw.fwetu, fw.rliqu, tu,
+                                R(P.rhoveg[ib][1]), R(P.tauveg[ib][1]), R(P.oriev[1]),
+                                R(P.orieh[1]), omegasno, tmelt, true, sunlit, fw.fwets, fw.rliqs,
+                                ts);
+    const TwoStr<R> up = twostr(coszen, lai[1], sai[1], asd, asi, su, eps, sunlit, ib == 0);
+    o.asurd[ib] = sunlit ? fu * up.refld + (R(1) - fu) * asd : R(0);
+    o.asuri[ib] = sunlit ? fu * up.refli + (R(1) - fu) * asi : R(0);
+    // solarf (radiation.py:450-536)
+    const R sd = solad[ib], si = solai[ib];
+    const R xx = sd * up.abvegd + si * up.abvegi;
+    const R y = lai[1] / mx(lai[1] + sai[1], eps);
+    const R solu = xx * y;
+    const R sols = xx * (R(1) - y);
+    const R sol2d = sd * up.fbeldd;
+    const R sol2i = sd * up.fbelid + si * up.fbelii;
+    const R sol3d = fu * sol2d + (R(1) - fu) * sd;
+    const R sol3i = fu * sol2i + (R(1) - fu) * si;
+    const R soll = sol3d * lo.abvegd + sol3i * lo.abvegi;
+    const R xd = (fl * lo.fbeldd + R(1) - fl) * sol3d;
+    const R xi = fl * (sol3d * lo.fbelid + sol3i * lo.fbelii) + (R(1) - fl) * sol3i;
+    const R solg = (R(1) - albsod) * xd + (R(1) - albsoi) * xi;
+    const R soli = (R(1) - albsnd) * sol3d + (R(1) - albsni) * sol3i;
+    if (sunlit) {
+      o.solu = o.solu + solu;
+      o.sols = o.sols + sols;
+      o.soll = o.soll + soll;
+      o.solg = o.solg + solg;
+      o.soli = o.soli + soli;
+    } else {
+      o.solu = o.solu + R(0);
+      o.sols = o.sols + R(0);
+      o.soll = o.soll + R(0);
+      o.solg = o.solg + R(0);
+      o.soli = o.soli + R(0);
+    }
+    if (ib == 0) {
+      const R xaiu = mx(lai[1] + sai[1], eps);
+      const R sc0 = up.term[0] * sd;
+      const R sc1 = up.term[1] * sd + up.term[2] * si;
+      const R sc2 = up.term[3] * sd + up.term[4] * si;
+      const R sc3 = sc0 + sc1 + sc2;
+      o.scalcoefu[0] = sc0; o.scalcoefu[1] = sc1; o.scalcoefu[2] = sc2; o.scalcoefu[3] = sc3;
+      o.topparu = sc3 * lai[1] / xaiu;
+      const R xail = mx(lai[0] + sai[0], eps);
+      const R l0 = lo.term[0] * sol3d;
+      const R l1 = lo.term[1] * sol3d + lo.term[2] * sol3i;
+      const R l2 = lo.term[3] * sol3d + lo.term[4] * sol3i;
+      const R l3 = l0 + l1 + l2;
+      o.scalcoefl[0] = l0; o.scalcoefl[1] = l1; o.scalcoefl[2] = l2; o.scalcoefl[3] = l3;
+      o.topparl = l3 * lai[0] / xail;
+      o.termu6 = up.term[5]; o.termu7 = up.term[6];
+      o.terml6 = lo.term[5]; o.terml7 = lo.term[6];
+    }
+  }
+}
+
+
+template <class D> struct IrOut { D firu, firs, firl, firg, firi, firb; };
+
+// irrad (radiation.py:543-619): net IR is a small difference of ~sigma T^4 terms, so the
+// solve precision D is used
+template <class D>
+IBIS_HD IBIS_INLINE void irrad(IrOut<D>& o, D fira, const D lai[2], const D sai[2], D fu, D fl,
+                               D fi, D tu, D ts, D tl, D tg, D ti, const IbisParams& P) {
+  const D stef = D(P.stef);
+  const D emu = D(1) - r_exp(-lai[1] / D(1));
+  const D ems = D(1) - r_exp(-sai[1] / D(1));
+  const D eml = D(1) - r_exp(-(lai[0] + sai[0]) / D(1));
+  const D emg = D(0.95), emi = D(0.95);
+  const D tu4 = stef * pow4(tu), ts4 = stef * pow4(ts), tl4 = stef * pow4(tl);
+  const D tg4 = stef * pow4(tg), ti4 = stef * pow4(ti);
+  const D fdown = (D(1) - fu) * fira +
+                  fu * ((D(1) - emu) * (D(1) - ems) * fira + emu * (D(1) - ems) * tu4 + ems * ts4);
+  const D fdowng = (D(1) - eml) * fdown + eml * tl4;
+  const D fupg = (D(1) - emg) * fdowng + emg * tg4;
+  const D fupgb = (D(1) - emg) * fdown + emg * tg4;
+  const D fupi = (D(1) - emi) * fdown + emi * ti4;
+  const D fup = (D(1) - fi) * (fl * (eml * tl4 + (D(1) - eml) * fupg) + (D(1) - fl) * fupgb) +
+                fi * fupi;
+  o.firb = (D(1) - fu) * fup + fu * ((D(1) - emu) * (D(1) - ems) * fup + emu * tu4 +
+                                     ems * (D(1) - emu) * ts4);
+  o.firu = emu * ems * ts4 + emu * (D(1) - ems) * fup + emu * fira - D(2) * emu * tu4;
+  o.firs = ems * emu * tu4 + ems * fup + ems * (D(1) - emu) * fira - D(2) * ems * ts4;
+  o.firl = eml * fdown + eml * fupg - D(2) * eml * tl4;
+  o.firg = fl * (fdowng - fupg) + (D(1) - fl) * (fdown - fupgb);
+  o.firi = fdown - fupi;
+}
+
+// ---------------------------------------------------------------------------------------
+// Interception (python-port/ibis/physics/interception.py)
+// ---------------------------------------------------------------------------------------
+
+// fwetcal (interception.py:62-94), one component
+template <class R>
+IBIS_HD IBIS_INLINE void fwet_one(R t, R wl, R ws, R wlmax, R wsmax, R fmax, R tmelt, R eps,
+                                  R& fwet, R& rliq) {
+  const R xliq = wl / mx(wlmax, R(0.01));
+  const R xtot = xliq + ws / mx(wsmax, R(0.01));
+  fwet = mn(fmax, xtot);
+  rliq = xliq / mx(xtot, eps);
+  if (fwet == R(0)) rliq = (t < tmelt - R(1.0e-9)) ? R(0) : R(1);
+}
+
+template <class R>
+IBIS_HD IBIS_INLINE Fwet<R> fwetcal(R tu, R ts, R tl, const R w[6], const IbisParams& P) {
+  Fwet<R> o;
+  const R fmax = R(P.fmax), tm = R(P.tmelt), eps = R(P.epsilon);
+  fwet_one(tu, w[0], w[3], R(P.wliqumax), R(P.wsnoumax), fmax, tm, eps, o.fwetu, o.rliqu);
+  fwet_one(ts, w[1], w[4], R(P.wliqsmax), R(P.wsnosmax), fmax, tm, eps, o.fwets, o.rliqs);
+  fwet_one(tl, w[2], w[5], R(P.wliqlmax), R(P.wsnolmax), fmax, tm, eps, o.fwetl, o.rliql);
+  return o;
+}
+
+// mix (interception.py:101-119)
+template <class R>
+IBIS_HD IBIS_INLINE void mix3(R x1, R t1, R x2, R t2, R x3, R t3, R eps, R& xm, R& tm) {
+  const R s = x1 + x2 + x3;
+  if (r_abs(s) >= eps) {
+    tm = (t1 * x1 + t2 * x2 + t3 * x3) / s;
+    xm = s;
+  } else {
+    tm = R(0);
+    xm = R(0);
+  }
+}
+
+// twet3 (interception.py:126-156): the latch freezes twk at the first converged step, so
+// the loop may stop there with the same result.
+template <class R> IBIS_HD IBIS_INLINE R twet3(R tak, R q, R p, const IbisParams& P) {
+  const R tmelt = R(P.tmelt);
+  const R qs0 = qsat(esat(tak), p);
+  const R ta = tak - tmelt;
+  R twk = ta * q / mx(qs0, R(1e-12)) + tmelt;
+  const R hvap = R(P.hvap), cvap = R(P.cvap), ch2o = R(P.ch2o), rcair = R(P.cair);
+  for (int it = 0; it < 20; ++it) {
+    const R twold = twk - tmelt;
+    const R hv = hvap + cvap * (tak - tmelt) - ch2o * (twk - tmelt);
+    const R qs = qsat(esat(twk), p);
+    const R twb = ta - (hv / rcair) * (qs - q);
+    const R nw = twold + R(0.2) * (twb - twold) + tmelt;
+    const bool conv = r_abs(nw - tmelt - twold) < R(0.02);
+    twk = nw;
+    if (conv) break;
+  }
+  return twk;
+}
+
+template <class R> struct H2O { R wliq, wsno, pflux, rain, train, snow, tsnow; };
+
+// steph2o (interception.py:174-235)
+template <class R>
+IBIS_HD IBIS_INLINE H2O<R> steph2o(R tveg, R wliq, R wsno, R xai, R rain, R train, R snow, R tsnow,
+                                   R tdrip, R tblow, R wliqmax, R wsnomax, R wliqmin, R wsnomin,
+                                   R dtime, const IbisParams& P) {
+  H2O<R> o;
+  const R eps = R(P.epsilon), tmelt = R(P.tmelt), ch2o = R(P.ch2o), cice = R(P.cice),
+          hfus = R(P.hfus);
+  const R fint = (xai >= eps) ? (R(1) - r_exp(R(-0.5) * xai)) / xai : R(0.5);
+  const R rw = R(1) / dtime;
+  R drip = xai * wliq / tdrip;
+  wliq = wliq * (R(1) - dtime / tdrip);
+  wliq = wliq + dtime * rain * fint;
+  R pflux = rain * fint * (tveg - train) * ch2o;
+  rain = rain * (R(1) - xai * fint);
+  R x = wliq;
+  wliq = mn(wliq, wliqmax);
+  if (wliq < wliqmin) wliq = R(0);
+  drip = drip + xai * (x - wliq) * rw;
+  R blow = xai * wsno / tblow;
+  wsno = wsno * (R(1) - dtime / tblow);
+  wsno = wsno + dtime * snow * fint;
+  pflux = pflux + snow * fint * (tveg - tsnow) * cice;
+  snow = snow * (R(1) - xai * fint);
+  x = wsno;
+  wsno = mn(wsno, wsnomax);
+  if (wsno < wsnomin) wsno = R(0);
+  blow = blow + xai * (x - wsno) * rw;
+  const R cw = R(P.ch2o - P.cice);
+  R dw = (tveg < tmelt - R(1.0e-9)) ? wliq : R(0);
+  pflux = pflux + dw * (cw * (tmelt - tveg) - hfus) * rw;
+  wliq = wliq - dw;
+  wsno = wsno + dw;
+  dw = (tveg > tmelt + R(1.0e-9)) ? wsno : R(0);
+  pflux = pflux + dw * (cw * (tveg - tmelt) + hfus) * rw;
+  wsno = wsno - dw;
+  wliq = wliq + dw;
+  mix3(rain, train, drip, tveg, R(0), R(0), eps, o.rain, o.train);
+  mix3(snow, tsnow, blow, tveg, R(0), R(0), eps, o.snow, o.tsnow);
+  o.wliq = wliq;
+  o.wsno = wsno;
+  o.pflux = pflux;
+  return o;
+}
+
+template <class R> struct Cascade {
+  R w[6];  // wliqu, wliqs, wliql, wsnou, wsnos, wsnol
+  R pfluxu, pfluxs, pfluxl, raing, traing, snowg, tsnowg, rainl, trainl, snowl, tsnowl;
+};
+
+// cascade (interception.py:263-357)
+template <class R>
+IBIS_HD void cascade(Cascade<R>& o, R tu, R ts, R tl, R t12, R t34, const R w[6], const R lai[2],
+                     const R sai[2], R fu, R fl, R ta, R qa, R psurf, R raina, R snowa, R dtime,
+                     const IbisParams& P) {
+  const R eps = R(P.epsilon), tmelt = R(P.tmelt);
+  const R wliqmin = R(0.001) * (dtime / R(3600)) * (R(P.wliqumax) / R(0.2));
+  const R wsnomin = R(0.001) * (dtime / R(3600)) * (R(P.wsnoumax) / R(2));
+  const R twb = (ta > tmelt + R(1.0e-9)) ? twet3(ta, qa, psurf, P) : tmelt;
+  R rainu, trainu, snowu, tsnowu;
+  mix3(raina, mx(twb, tmelt), R(0), mx(t12, tmelt), R(0), R(0), eps, rainu, trainu);
+  mix3(snowa, mn(ta, tmelt), R(0), mn(t12, tmelt), R(0), R(0), eps, snowu, tsnowu);
+  const H2O<R> u = steph2o(tu, w[0], w[3], R(2) * lai[1], rainu, trainu, snowu, tsnowu,
+                           R(P.tdripu), R(P.tblowu), R(P.wliqumax), R(P.wsnoumax), wliqmin,
+                           wsnomin, dtime, P);
+  const H2O<R> s = steph2o(ts, w[1], w[4], R(2) * sai[1], u.rain, u.train, u.snow, u.tsnow,
+                           R(P.tdrips), R(P.tblows), R(P.wliqsmax), R(P.wsnosmax), wliqmin,
+                           wsnomin, dtime, P);
+  mix3(fu * s.rain, s.train, (R(1) - fu) * rainu, trainu, R(0), mx(t34, tmelt), eps, o.rainl,
+       o.trainl);
+  mix3(fu * s.snow, s.tsnow, (R(1) - fu) * snowu, tsnowu, R(0), mn(t34, tmelt), eps, o.snowl,
+       o.tsnowl);
+  const H2O<R> l = steph2o(tl, w[2], w[5], R(2) * (lai[0] + sai[0]), o.rainl, o.trainl, o.snowl,
+                           o.tsnowl, R(P.tdripl), R(P.tblowl), R(P.wliqlmax), R(P.wsnolmax),
+                           wliqmin, wsnomin, dtime, P);
+  mix3(fl * l.rain, l.train, (R(1) - fl) * o.rainl, o.trainl, R(0), R(0), eps, o.raing, o.traing);
+  mix3(fl * l.snow, l.tsnow, (R(1) - fl) * o.snowl, o.tsnowl, R(0), R(0), eps, o.snowg, o.tsnowg);
+  o.w[0] = u.wliq; o.w[1] = s.wliq; o.w[2] = l.wliq;
+  o.w[3] = u.wsno; o.w[4] = s.wsno; o.w[5] = l.wsno;
+  o.pfluxu = u.pflux; o.pfluxs = s.pflux; o.pfluxl = l.pflux;
+}
+
+// steph2o2 (interception.py:364-416)
+template <class R>
+IBIS_HD IBIS_INLINE void steph2o2(R& tveg, R& wliq, R& wsno, R fveg, R xai, R rliq, R fvapw,
+                                  R cveg, R dtime, R ta, R& fvapa, R& fsena, const IbisParams& P) {
+  const R tmelt = R(P.tmelt), hfus = R(P.hfus), ch2o = R(P.ch2o), cice = R(P.cice);
+  wliq = wliq - dtime * rliq * fvapw;
+  wsno = wsno - dtime * (R(1) - rliq) * fvapw;
+  const bool has = fveg * xai > R(0);
+  R zm = has ? mx(-wliq, R(0)) * fveg * xai / dtime : R(0);
+  fvapa = fvapa + zm;
+  fsena = fsena - zm * (R(P.hvap) + R(P.cvap) * (ta - tmelt) - ch2o * (tveg - tmelt));
+  wliq = mx(wliq, R(0));
+  zm = has ? mx(-wsno, R(0)) * fveg * xai / dtime : R(0);
+  fvapa = fvapa + zm;
+  fsena = fsena - zm * (R(P.hsub) + R(P.cvap) * (ta - tmelt) - cice * (tveg - tmelt));
+  wsno = mx(wsno, R(0));
+  const R rw = R(1) / hfus;
+  if ((tveg < tmelt - R(1.0e-9)) && (wliq > R(0))) {
+    const R chav = cveg + ch2o * wliq + cice * wsno;
+    const R dh = chav * (tmelt - tveg);
+    const R dw = mn(wliq, mx(R(0), dh * rw));
+    wliq = wliq - dw;
+    wsno = wsno + dw;
+    const R chn = cveg + ch2o * wliq + cice * wsno;
+    tveg = tmelt - (dh - hfus * dw) / chn;
+  }
+  if ((tveg > tmelt + R(1.0e-9)) && (wsno > R(0))) {
+    const R chav = cveg + ch2o * wliq + cice * wsno;
+    const R dh = chav * (tveg - tmelt);
+    const R dw = mn(wsno, mx(R(0), dh * rw));
+    wsno = wsno - dw;
+    wliq = wliq + dw;
+    const R chn = cveg + ch2o * wliq + cice * wsno;
+    tveg = tmelt + (dh - hfus * dw) / chn;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Aerodynamics (python-port/ibis/physics/canopy_exchange.py)
+// ---------------------------------------------------------------------------------------
+template <class R> struct Aero {
+  R rhoa, cp, za, z1, z2, z3, z4, z12, z34, bdl, bdu, dil, diu, exphl, expl, exphu, expu;
+  R alogav, alog1, alog2, alog3, alog4, aloga, alogu, alogl, u2;
+};
+
+// canini (canopy_exchange.py:192-280); tfac is a host constant
+template <class R>
+IBIS_HD void canini(Aero<R>& a, R ta, R qa, R psurf, R ua, R fi, R z0soi, R fl, R fu,
+                    const R lai[2], const R sai[2], const R ztop[2], const R zbot[2],
+                    const IbisParams& P) {
+  const R siga = R(0.999);
+  const R pa = psurf * siga;
+  a.rhoa = pa / (R(P.rair) * ta * (R(1) + R(P.rvap / P.rair - 1.0) * qa));
+  a.cp = R(P.cair) * (R(1) + R(P.cvap / P.cair - 1.0) * qa);
+  R za = (psurf - pa) / (a.rhoa * R(P.grav));
+  a.za = mx(za, ztop[1] + R(1));
+  const R z0sno = R(P.z0sno);
+  R x = fl * (R(1) - fi) * R(2) * (lai[0] + sai[0]) / R(P.alaiml);
+  x = mn(x, R(3));
+  R x1 = mn(x, R(1));
+  R rw = mx(ztop[0] - zbot[0], R(0.01));
+  const R cvegl = (R(0.4) / rw) * mx(R(1e-5), x);
+  const R dvegl = (R(0.1) * rw) / mx(R(1e-5), mx(x, x * x));
+  const R bvegl = r_sqrt(R(2) * cvegl / dvegl);
+  a.bdl = R(0.5) * bvegl * dvegl;
+  a.dil = R(1) / dvegl;
+  rw = (R(1) - x1) * (mx(z0soi, z0sno) + R(0.01));
+  a.z3 = x1 * ztop[0] + rw;
+  a.z4 = x1 * zbot[0] + rw;
+  a.z34 = R(0.5) * (a.z3 + a.z4);
+  a.exphl = r_exp(R(0.5) * bvegl * (a.z3 - a.z4));
+  a.expl = a.exphl * a.exphl;
+  const R displ = x1 * R(0.7) * a.z3;
+  x = fu * R(2) * (lai[1] + sai[1]) / R(P.alaimu);
+  x = mn(x, R(3));
+  x1 = mn(x, R(1));
+  rw = mx(ztop[1] - zbot[1], R(0.01));
+  const R cvegu = (R(0.4) / rw) * mx(R(1e-5), x);
+  const R dvegu = (R(0.1) * rw) / mx(R(1e-5), mx(x, x * x));
+  rw = R(1) / dvegu;
+  const R bvegu = r_sqrt(R(2) * cvegu * rw);
+  a.bdu = R(0.5) * bvegu * dvegu;
+  a.diu = rw;
+  rw = (R(1) - x1) * (a.z3 + R(0.01));
+  a.z1 = x1 * ztop[1] + rw;
+  a.z2 = x1 * zbot[1] + rw;
+  a.z12 = R(0.5) * (a.z1 + a.z2);
+  a.exphu = r_exp(R(0.5) * bvegu * (a.z1 - a.z2));
+  a.expu = a.exphu * a.exphu;
+  const R dispu = x1 * R(0.7) * a.z1 + (R(1) - x1) * displ;
+  const R alogg = r_log(z0soi);
+  const R alogi = r_log(z0sno);
+  a.alogav = (R(1) - fi) * alogg + fi * alogi;
+  a.alog4 = r_log(mx(a.z4, mx(R(1.1) * z0soi, R(1.1) * z0sno)));
+  a.alog3 = r_log(a.z3 - displ);
+  a.alog2 = r_log(a.z2 - displ);
+  a.alog1 = r_log(a.z1 - dispu);
+  a.aloga = r_log(a.za - dispu);
+  a.u2 = ua / a.exphu;
+  a.alogu = r_log(mx(R(0.01), R(0.1) * (a.z1 - a.z2)));
+  a.alogl = r_log(mx(R(0.01), R(0.1) * (a.z3 - a.z4)));
+}
+
+// fstrat (canopy_exchange.py:287-326); returns stram/strah blended with the previous pass
+template <class R>
+IBIS_HD IBIS_INLINE void fstrat(R tb, R tt, R ttfac, R qb, R qt, R zb, R zt, R albm, R albh, R alt,
+                                R u, R grav, R vonk, bool first, R& stram, R& strah) {
+  const R zht = tt * ttfac * (R(1) + R(0.622) * qt);
+  const R zhb = tb * (R(1) + R(0.622) * qb);
+  R rich = grav * mx(zt - zb, R(0)) * (zht - zhb) / (R(0.5) * (zht + zhb) * (u * u));
+  rich = clip(rich, R(-2), R(1));
+  R sm, sh;
+  if (rich <= R(0)) {
+    const R xm = mx(alt - albm, R(0.5));
+    const R xh = mx(alt - albh, R(0.5));
+    const R rw = r_sqrt(mx(-rich, R(0)));
+    const R ym = ((vonk / xm) * (vonk / xm)) * r_exp(R(0.5) * xm) * rw;
+    const R yh = ((vonk / xh) * (vonk / xh)) * r_exp(R(0.5) * xh) * rw;
+    sm = R(1) - R(10) * rich / (R(1) + R(75) * ym);
+    sh = R(1) - R(15) * rich / (R(1) + R(75) * yh);
+  } else {
+    const R rp = mx(rich, R(0));
+    const R z = r_sqrt(R(1) + R(5) * rp);
+    sm = R(1) / (R(1) + R(10) * rp / z);
+    sh = R(1) / (R(1) + R(15) * rp * z);
+  }
+  if (first) {
+    stram = sm;
+    strah = sh;
+  } else {
+    stram = R(0.5) * sm + R(0.5) * stram;
+    strah = R(0.5) * sh + R(0.5) * strah;
+  }
+}
+
+template <class R> struct Turcof { R cu, cl, su, ss, sl, sg, si; };
+
+// turcof (canopy_exchange.py:333-426); u2/alogu/alogl/strat are carried between passes
+template <class R>
+IBIS_HD void turcof(Turcof<R>& o, const Aero<R>& a, R tfac, R ta, R qa, R ua, R t12, R t34, R q12,
+                    R q34, R dtime, bool first, R& u2c, R& aloguc, R& aloglc, R& straml,
+                    R& strahl, R& stramu, R& strahu, const IbisParams& P) {
+  const R vonk = R(P.vonk), grav = R(P.grav);
+  fstrat(t34, t12, R(1), q34, q12, a.z3, a.z2, aloglc, aloglc, a.alog2, u2c, grav, vonk, first,
+         straml, strahl);
+  fstrat(t12, ta, tfac, q12, qa, a.z1, a.za, aloguc, aloguc, a.aloga, ua, grav, vonk, first,
+         stramu, strahu);
+  R x = (a.alog4 - a.alogav) / vonk;
+  x = (x * x) * a.bdl;
+  R rw = R(1) / a.expl;
+  const R yl = ((x + R(1)) * a.expl + (x - R(1)) * rw) / ((x + R(1)) * a.expl - (x - R(1)) * rw);
+  const R alogl = a.alog3 - vonk * r_sqrt(yl / a.bdl);
+  x = (a.alog2 - alogl) / vonk;
+  x = (x * x) * a.bdu / straml;
+  rw = R(1) / a.expu;
+  const R yu = ((x + R(1)) * a.expu + (x - R(1)) * rw) / ((x + R(1)) * a.expu - (x - R(1)) * rw);
+  const R alogu = a.alog1 - vonk * r_sqrt(yu / a.bdu);
+  const R cdmax = R(300) / (R(2) * dtime);
+  const R t = ua * vonk / (a.aloga - alogu);
+  const R tauu = (t * t) * stramu;
+  const R aa = R(0.5) * tauu * (yu + R(1)) / a.bdu;
+  const R bb = R(0.5) * tauu * (yu - R(1)) / a.bdu;
+  const R taul = a.bdu * (aa / a.expu - bb * a.expu);
+  const R cc = R(0.5) * taul * (yl + R(1)) / a.bdl;
+  const R dd = R(0.5) * taul * (yl - R(1)) / a.bdl;
+  const R u1 = mx(R(0.01), r_sqrt(mx(R(0), aa + bb)));
+  const R u12 = mx(R(0.01), r_sqrt(mx(R(0), aa / a.exphu + bb * a.exphu)));
+  const R u2 = mx(R(0.01), r_sqrt(mx(R(0), aa / a.expu + bb * a.expu)));
+  const R u3 = mx(R(0.01), r_sqrt(mx(R(0), cc + dd)));
+  const R u34 = mx(R(0.01), r_sqrt(mx(R(0), cc / a.exphl + dd * a.exphl)));
+  const R u4 = mx(R(0.01), r_sqrt(mx(R(0), cc / a.expl + dd * a.expl)));
+  R ca = ua * strahu * (vonk * vonk) / ((a.aloga - alogu) * (a.aloga - a.alog1));
+  ca = mn(cdmax, ca / (R(1) + ca * R(1e-20)));
+  const R cai = R(1) / (a.rhoa * ca);
+  const R cbi = a.diu * (a.z1 - a.z12) / (a.rhoa * R(0.5) * (u1 + u12));
+  const R cci = a.diu * (a.z12 - a.z2) / (a.rhoa * R(0.5) * (u12 + u2));
+  const R cdi = (a.alog2 - alogl) * (a.alog2 - a.alog3) / (a.rhoa * u2 * strahl * (vonk * vonk));
+  const R cei = a.dil * (a.z3 - a.z34) / (a.rhoa * R(0.5) * (u3 + u34));
+  const R cfi = a.dil * (a.z34 - a.z4) / (a.rhoa * R(0.5) * (u34 + u4));
+  o.cu = R(1) / (cai + cbi);
+  o.cl = R(1) / (cci + cdi + cei);
+  o.su = a.rhoa * R(P.cleaf) * r_sqrt(u12 / R(P.dleaf[1]));
+  o.ss = a.rhoa * R(P.cstem) * r_sqrt(u12 / R(P.dstem[1]));
+  o.sl = a.rhoa * R(P.cgrass) * r_sqrt(u34 / R(P.dleaf[0]));
+  const R sg0 = a.rhoa * (R(0.004) + R(0.012) * u4);
+  const R si0 = a.rhoa * (R(0.003) + R(0.010) * u4);
+  o.sg = R(1) / (cfi + R(1) / sg0);
+  o.si = R(1) / (cfi + R(1) / si0);
+  u2c = u2;
+  aloguc = alogu;
+  aloglc = alogl;
+}
+
+// impexp (canopy_exchange.py:433-445); not called on the first pass (weight 1 there)
+template <class R>
+IBIS_HD IBIS_INLINE R impexp(R tveg, R ch, R wliq, R wsno, R prev, const IbisParams& P) {
+  const R eps = R(P.epsilon), hfus = R(P.hfus);
+  const R h = ch * (tveg - R(P.tmelt));
+  const R z = mx(r_abs(h), eps);
+  R wn;
+  if (h > eps + R(1.0e-8)) wn = R(1) - mn(R(1), hfus * wsno / z);
+  else if (h < -eps - R(1.0e-8)) wn = R(1) - mn(R(1), hfus * wliq / z);
+  else wn = R(1);
+  return R(0.5) * (prev + wn);
+}
+
+// impexp2 (canopy_exchange.py:448-463)
+template <class R> IBIS_HD IBIS_INLINE R impexp2(R t, R told, R prev, const IbisParams& P) {
+  const R dt = t - told;
+  const R wn = (dt > R(P.epsilon) + R(1.0e-8)) ? (R(P.tmelt) - told) / dt : prev;
+  return clip(wn, R(0), R(1));
+}
+
+}  // namespace ibis

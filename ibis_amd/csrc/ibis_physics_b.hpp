@@ -1,0 +1,456 @@
+// Per-cell hourly physics, part B: photosynthesis/stomata, the 9x9 canopy energy and
+// vapour solve, snow, soil water and heat with bedrock, fire accumulators.
+//
+// D is the precision of the implicit solves (the 9x9 canopy system and the snow, soil and
+// bedrock tridiagonals with their assembly); R is the bulk precision.
+#pragma once
+
+#include "ibis_physics_a.hpp"
+
+// Trace point for the assembled canopy system in host-side debugging builds.
+#ifndef IBIS_TRACE_SYSTEM
+#define IBIS_TRACE_SYSTEM(A, b) ((void)0)
+#endif
+
+namespace ibis {
+
+// ---------------------------------------------------------------------------------------
+// Stomata (python-port/ibis/physics/stomata.py), F77 switches
+// ---------------------------------------------------------------------------------------
+
+// colimit (stomata.py:198-221): smaller root of theta x^2 - (j1 + j2) x + j1 j2
+template <class R> IBIS_HD IBIS_INLINE R colimit(R j1, R j2, R theta) {
+  const R b = j1 + j2;
+  const R c = j1 * j2;
+  const R disc = mx(b * b - R(4) * theta * c, R(0));
+  const R q = R(0.5) * (b + r_sqrt(disc)) + R(1.0e-15);
+  return mn(q / theta, c / q);
+}
+
+template <class R> struct Leaf { R ag, an; };
+
+// farquhar_c3 (stomata.py:228-272)
+template <class R>
+IBIS_HD IBIS_INLINE Leaf<R> farq_c3(R par, R ci, R vmax, R gstar, R kc, R ko, R o2, R rdark,
+                                    const IbisParams& P) {
+  const R je = par * R(4.59e-06) * R(P.alpha3) * (ci - gstar) / (ci + R(2) * gstar);
+  const R jc = vmax * (ci - gstar) / (ci + kc * (R(1) + o2 / ko));
+  const R js = vmax / R(2.2);
+  Leaf<R> o;
+  o.ag = colimit(colimit(je, jc, R(P.theta3)), js, R(P.beta3));
+  o.an = o.ag - rdark;
+  return o;
+}
+
+// farquhar_c4 (stomata.py:279-314)
+template <class R>
+IBIS_HD IBIS_INLINE Leaf<R> farq_c4(R par, R ci, R vmax, R rdark, const IbisParams& P) {
+  const R je = par * R(4.59e-06) * R(P.alpha4);
+  const R ji = R(18.0e3) * vmax * ci;
+  Leaf<R> o;
+  o.ag = colimit(colimit(je, vmax, R(P.theta4)), ji, R(P.beta4));
+  o.an = o.ag - rdark;
+  return o;
+}
+
+// ball_berry_step (stomata.py:321-372); cs/gs/ci updated in place
+template <class R>
+IBIS_HD IBIS_INLINE void ball_berry(R an, R& cs, R& gs, R& ci, R co2, R gb, R rh, R coefm,
+                                    R coefb, R stress, R gsmin, R gmin, R cimax, bool c4) {
+  R c = R(0.5) * (cs + co2 - an / gb);
+  c = mx(c4 ? R(1.0e-8) : gmin, c);
+  R g = R(0.5) * (gs + coefm * an * rh / c + coefb * stress);
+  g = mx(g, mx(gsmin, coefb * stress));
+  R i = R(0.5) * (ci + c - R(1.6) * an / g);
+  i = clip(i, c4 ? R(0) : gmin, cimax);
+  cs = c;
+  gs = g;
+  ci = i;
+}
+
+// canopy_scaling (stomata.py:442-508), F77 denominator
+template <class R>
+IBIS_HD IBIS_INLINE R canopy_scaling(R toppar, R t6, R t7, const R sc[4], R lai, R sai, R& a10s,
+                                     R& a10d, R zw, R eps) {
+  R ext = (t6 * sc[0] + t7 * sc[1] - t7 * sc[2]) / mx(sc[3], eps);
+  ext = clip(ext, R(0.1), R(10));
+  const R pxai = ext * (lai + sai);
+  const R plai = ext * lai;
+  const bool has = plai > R(0);
+  const bool day = toppar > R(10);
+  const R sday = (R(1) - r_exp(-pxai)) / mx(plai, eps);
+  const R snight = a10s / mx(a10d, eps);
+  if (has && day) {
+    a10s = zw * a10s + (R(1) - zw) * sday * toppar;
+    a10d = zw * a10d + (R(1) - zw) * toppar;
+  }
+  return has ? (day ? sday : snight) : R(0);
+}
+
+template <class R> struct TempDep { R kc, ko, gstar, tv3, tv4; };
+
+// _temp_dep_params (stomata.py:661-689)
+template <class R> IBIS_HD IBIS_INLINE TempDep<R> temp_dep(R tleaf, const IbisParams& P) {
+  TempDep<R> o;
+  const R rw = R(3.47e-03) - R(1) / tleaf;
+  const R tau = R(P.tau15) * r_exp(R(-4500) * rw);
+  o.kc = R(P.kc15) * r_exp(R(6000) * rw);
+  o.ko = R(P.ko15) * r_exp(R(1500) * rw);
+  o.gstar = R(P.o2conc) / (R(2) * tau);
+  const R tc = tleaf - R(273.16);
+  const R base = r_exp(R(3500) * rw);
+  const R cold3 = R(1) + r_exp(R(0.40) * (R(5) - tc));
+  const R cold4 = R(1) + r_exp(R(0.40) * (R(10) - tc));
+  const R hot = R(1) + r_exp(R(0.40) * (tc - R(50)));
+  o.tv3 = base / (cold3 * hot);
+  o.tv4 = base / (cold4 * hot);
+  return o;
+}
+
+// _total_conductance (stomata.py:696-725), F77 order
+template <class R> IBIS_HD IBIS_INLINE R total_cond(R su, R gsc) {
+  return R(1) / (R(1) / su + R(1.0 / 0.029) / gsc);
+}
+
+template <class R> struct Stom {
+  R ci[5], gs[5], cs[5];   // ub, uc, ls, l3, l4
+  R an[5], ag[5];
+  R tcond[5];
+  R a10[4];                // scalparamu, daylightu, scalparaml, daylightl
+};
+
+// stomata (stomata.py:732-1080); vmax6 = (pft 0, 2, 4, 3, 5, 8, 10, 11) nominal values
+template <class R>
+IBIS_HD void stomata(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R sl, R psurf,
+                     const RadOut<R>& rad, R fwetu, R fwetl, const R lai[2], const R sai[2],
+                     R vub, R vuc, R vls, R vl3, R vl4, R stresstu, R stresstl, R co2, R zw,
+                     const IbisParams& P) {
+  const R eps = R(P.epsilon), cimax = R(P.cimax), o2 = R(P.o2conc);
+  // upper canopy
+  TempDep<R> td = temp_dep(tu, P);
+  R gmin = R(1.05) * td.gstar;
+  R gb = clip(su * R(25.5), R(0.1), R(10));
+  const R rh12 = mx(R(0.30), q12 / mx(qsat(esat(t12), psurf), eps));
+  const R vnom_u[2] = {vub, vuc};
+  Leaf<R> lf[5];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    s.ci[g] = clip(s.ci[g], gmin, cimax);
+    const R vmax = vnom_u[g] * td.tv3 * stresstu;
+    const R rd = R(P.gamma[g]) * vnom_u[g] * td.tv3;
+    lf[g] = farq_c3(rad.topparu, s.ci[g], vmax, td.gstar, td.kc, td.ko, o2, rd, P);
+    ball_berry(lf[g].an, s.cs[g], s.gs[g], s.ci[g], co2, gb, rh12, R(P.coefm[g]), R(P.coefb[g]),
+               stresstu, R(P.gsmin[g]), gmin, cimax, false);
+  }
+  R scale = canopy_scaling(rad.topparu, rad.termu6, rad.termu7, rad.scalcoefu, lai[1], sai[1],
+                           s.a10[0], s.a10[1], zw, eps);
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const R agc = lf[g].ag * scale;
+    const R anc = lf[g].an * scale;
+    const R csc = mx(gmin, co2 - anc / gb);
+    const R gsc = mx(mx(R(P.gsmin[g]), R(P.coefb[g]) * stresstu),
+                     R(P.coefm[g]) * anc * rh12 / csc + R(P.coefb[g]) * stresstu);
+    s.tcond[g] = total_cond(su, gsc);
+    s.ag[g] = agc * (R(1) - fwetu);
+    s.an[g] = anc * (R(1) - fwetu);
+  }
+  // lower canopy: shrub (group 2), C3 grass (group 4), C4 grass (group 3)
+  td = temp_dep(tl, P);
+  gmin = R(1.05) * td.gstar;
+  gb = clip(sl * R(25.5), R(0.1), R(10));
+  const R rh34 = mx(R(0.30), q34 / mx(qsat(esat(t34), psurf), eps));
+  // slot k of the lower arrays -> Ball-Berry parameter group
+  const int grp[3] = {2, 4, 3};
+  const R vnom_l[3] = {vls, vl3, vl4};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int g = grp[k];
+    const bool c4 = (k == 2);
+    s.ci[2 + k] = clip(s.ci[2 + k], c4 ? R(0) : gmin, cimax);
+    const R tv = c4 ? td.tv4 : td.tv3;
+    const R vmax = vnom_l[k] * tv * stresstl;
+    const R rd = R(P.gamma[g]) * vnom_l[k] * tv;
+    lf[2 + k] = c4 ? farq_c4(rad.topparl, s.ci[2 + k], vmax, rd, P)
+                   : farq_c3(rad.topparl, s.ci[2 + k], vmax, td.gstar, td.kc, td.ko, o2, rd, P);
+    ball_berry(lf[2 + k].an, s.cs[2 + k], s.gs[2 + k], s.ci[2 + k], co2, gb, rh34,
+               R(P.coefm[g]), R(P.coefb[g]), stresstl, R(P.gsmin[g]), gmin, cimax, c4);
+  }
+  scale = canopy_scaling(rad.topparl, rad.terml6, rad.terml7, rad.scalcoefl, lai[0], sai[0],
+                         s.a10[2], s.a10[3], zw, eps);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int g = grp[k];
+    const R agc = lf[2 + k].ag * scale;
+    const R anc = lf[2 + k].an * scale;
+    const R csc = mx(k == 2 ? R(1.0e-8) : gmin, co2 - anc / gb);
+    const R gsc = mx(mx(R(P.gsmin[g]), R(P.coefb[g]) * stresstl),
+                     R(P.coefm[g]) * anc * rh34 / csc + R(P.coefb[g]) * stresstl);
+    s.tcond[2 + k] = total_cond(sl, gsc);
+    s.ag[2 + k] = agc * (R(1) - fwetl);
+    s.an[2 + k] = anc * (R(1) - fwetl);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// 9x9 canopy system (canopy_exchange.py:470-811)
+// ---------------------------------------------------------------------------------------
+
+// Structural non-zeros of the canopy system after no-pivot elimination fill-in (bit j of
+// row i).  Entries outside the pattern are exactly zero throughout, so skipping them
+// reproduces the dense elimination of canopy_exchange.py:26-61 for finite values.
+IBIS_HD constexpr int nz9_row(int i) {
+  return i == 0 ? 0x29 : i == 1 ? 0x2a : i == 2 ? 0x54 : i == 3 ? 0x3b : i == 4 ? 0x1fc
+       : i == 5 ? 0x1fb : i == 6 ? 0x1f4 : 0x1f0;
+}
+#define IBIS_NZ9(i, j) ((nz9_row(i) >> (j)) & 1)
+
+// No-pivot Gaussian elimination in F77 linsolve order (math.f:73), then back-substitution.
+template <class D> IBIS_HD IBIS_INLINE void gauss9(D A[9][9], D b[9], D x[9]) {
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+#pragma unroll
+    for (int i = p + 1; i < 9; ++i) {
+      if (IBIS_NZ9(i, p)) {
+        const D f = A[i][p] / A[p][p];
+#pragma unroll
+        for (int j = p + 1; j < 9; ++j)
+          if (IBIS_NZ9(p, j)) A[i][j] = A[i][j] - f * A[p][j];
+        b[i] = b[i] - f * b[p];
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 8; p >= 0; --p) {
+    D s = D(0);
+    bool first = true;
+#pragma unroll
+    for (int j = p + 1; j < 9; ++j)
+      if (IBIS_NZ9(p, j)) {
+        s = first ? A[p][j] * x[j] : s + A[p][j] * x[j];
+        first = false;
+      }
+    x[p] = (b[p] - s) / A[p][p];
+  }
+}
+
+// Inputs of turvap that stay fixed over the three passes.
+template <class D> struct TvFix {
+  D tuold, tsold, tlold, tgold, tiold;
+  D cp, tfac, xu, xs, xl, fi, fu, fl, lai0, lai1, sai0, sai1;
+  D frac[12];
+  D chux, chsx, chlx, w[6];
+  D rliqu, rliqs, rliql, fwetu, fwets, fwetl;
+  D cog, coi, zirg, ziri, wpud, wipud, wisoi1, wsoi1, swilt1, suction1, bex1, hvasug, hvasui;
+  D tsoi1, tsno1, solu, sols, soll, solg, soli, firu, firs, firl, firg, firi, firb;
+  D pfluxu, pfluxs, pfluxl, ta, qa, psurf;
+};
+
+template <class D> struct TvOut {
+  D t[9];  // tu ts tl t12 t34 q12 q34 tg ti
+  D fsena, fsenu, fsens, fsenl, fseng, fseni;
+  D fvapa, fvapuw, fvaput, fvaps, fvaplw, fvaplt, fvapg, fvapi;
+  D firb, firg, firi, ginvap, gsuvap, gtrans;
+};
+
+// turvap (canopy_exchange.py:470-811)
+template <class D>
+IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, D su, D ss, D sl,
+                    D sg, D si, const D tc[5], D wu, D ws, D wl, D wg, D wi, D dtime,
+                    const IbisParams& P) {
+  const D eps = D(P.epsilon), tmelt = D(P.tmelt);
+  const D tu = wu * cur[0] + (D(1) - wu) * f.tuold;
+  const D ts = ws * cur[1] + (D(1) - ws) * f.tsold;
+  const D tl = wl * cur[2] + (D(1) - wl) * f.tlold;
+  const D t34 = cur[4], q12 = cur[5], q34 = cur[6];
+  const D tg = wg * cur[7] + (D(1) - wg) * f.tgold;
+  const D ti = wi * cur[8] + (D(1) - wi) * f.tiold;
+  const D ps = f.psurf;
+  D e = esat(tu);
+  const D qu = qsat(e, ps), dqu = mn(dqsat(tu, qu, e), qu * D(0.1));
+  e = esat(ts);
+  const D qs = qsat(e, ps), dqs = mn(dqsat(ts, qs, e), qs * D(0.1));
+  e = esat(tl);
+  const D ql = qsat(e, ps), dql = mn(dqsat(tl, ql, e), ql * D(0.1));
+  e = esat(tg);
+  const D qg = qsat(e, ps), dqg = mn(dqsat(tg, qg, e), qg * D(0.1));
+  e = esat(ti);
+  const D qi = qsat(e, ps), dqi = mn(dqsat(ti, qi, e), qi * D(0.1));
+
+  const D zwpud = clip(D(0.5) * (f.wpud + f.wipud) / mx(D(P.wpudmax), eps), D(0), D(0.5));
+  const D zwsoi = (D(1) - f.wisoi1) * f.wsoi1 + f.wisoi1;
+  const D zwtot = zwpud + (D(1) - zwpud) * zwsoi;
+  const D psig = -D(P.grav) * f.suction1 * r_pow(zwtot, -f.bex1);
+  const D hfac = r_exp(psig / (D(P.rvap) * tg));
+  const D hfac2 = q34 / mx(qg, eps);
+  D betaw = clip((zwtot - f.swilt1) / mx(D(1) - f.swilt1, eps), D(0), D(1));
+  if (P.soil_beta_power != 1.0) betaw = r_pow(betaw, D(P.soil_beta_power));
+  const D e34 = esat(t34);
+  const D qs1 = qsat(e34, ps);
+  const D dqs1 = dqsat(t34, qs1, e34);
+  const D betafac = D(P.hvap) * dqs1 / (f.cp + (D(4.0 * 0.95) * D(P.stef) * cube(t34)) / sg);
+  const D betas = betaw / (D(1) + betafac * (D(1) - betaw));
+  const D qgfac0 = (betas > D(0)) ? betas * hfac + (D(1) - betas) * hfac2 : hfac2;
+  const D qeps = D(1.0e-10);
+  const D qgfac = (q34 > qg + qeps) ? D(1) : qgfac0;
+  const D fwetux = (q12 > qu + qeps) ? D(1) : f.fwetu;
+  const D fwetsx = (q12 > qs + qeps) ? D(1) : f.fwets;
+  const D fwetlx = (q34 > ql + qeps) ? D(1) : f.fwetl;
+  const D fradu = (f.lai1 > eps) ? (f.solu + f.firu) / (D(2) * f.lai1) : D(0);
+  const D frads = (f.sai1 > eps) ? (f.sols + f.firs) / (D(2) * f.sai1) : D(0);
+  const D fradl = ((f.lai0 + f.sai0) > eps) ? (f.soll + f.firl) / (D(2) * (f.lai0 + f.sai0)) : D(0);
+
+  const D ta = f.ta;
+  const D hvc = D(P.hvap), hsc = D(P.hsub), cvc = D(P.cvap), chc = D(P.ch2o), cic = D(P.cice);
+  const D hvu = hvc + cvc * (ta - tmelt) - chc * (tu - tmelt);
+  const D hsu = hsc + cvc * (ta - tmelt) - cic * (tu - tmelt);
+  const D hvs = hvc + cvc * (ta - tmelt) - chc * (ts - tmelt);
+  const D hss = hsc + cvc * (ta - tmelt) - cic * (ts - tmelt);
+  const D hvl = hvc + cvc * (ta - tmelt) - chc * (tl - tmelt);
+  const D hsl = hsc + cvc * (ta - tmelt) - cic * (tl - tmelt);
+  const D* fr = f.frac;
+  const D tub = tc[0], tuc = tc[1], tls = tc[2], tl3 = tc[3], tl4 = tc[4];
+  const D suw = mn(fwetux * su, D(0.8) * (f.w[0] + f.w[3]) / mx(dtime * (qu - q12), eps));
+  const D sut = mx(D(0), (D(1) - fwetux) * D(0.5) *
+                              (tub * fr[0] + tub * fr[1] + tub * fr[2] + tuc * fr[3] +
+                               tub * fr[4] + tuc * fr[5] + tub * fr[6] + tub * fr[7]));
+  const D suh = suw * (f.rliqu * hvu + (D(1) - f.rliqu) * hsu) + sut * hvu;
+  const D ssw = mn(fwetsx * ss, D(0.8) * (f.w[1] + f.w[4]) / mx(dtime * (qs - q12), eps));
+  const D ssh = ssw * (f.rliqs * hvs + (D(1) - f.rliqs) * hss);
+  const D slw = mn(fwetlx * sl, D(0.8) * (f.w[2] + f.w[5]) / mx(dtime * (ql - q34), eps));
+  const D slt0 = mx(D(0), (D(1) - fwetlx) * D(0.5) *
+                              (tls * fr[8] + tls * fr[9] + tl4 * fr[10] + tl3 * fr[11]));
+  const D slt = slt0 * f.lai0 / mx(f.lai0 + f.sai0, eps);
+  const D slh = slw * (f.rliql * hvl + (D(1) - f.rliql) * hsl) + slt * hvl;
+
+  D A[9][9];
+  D b[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) A[i][j] = D(0);
+    b[i] = D(0);
+  }
+  const D rw = D(1) / dtime;
+  const D cp = f.cp, fi = f.fi, xu = f.xu, xs = f.xs, xl = f.xl;
+  D w2 = su * cp;
+  A[0][0] = f.chux * rw + wu * w2 + wu * suh * dqu;
+  A[0][3] = -w2;
+  A[0][5] = -suh;
+  b[0] = f.tuold * f.chux * rw - (D(1) - wu) * w2 * tu - suh * (qu - wu * dqu * tu) + fradu -
+         f.pfluxu;
+  w2 = ss * cp;
+  A[1][1] = f.chsx * rw + ws * w2 + ws * ssh * dqs;
+  A[1][3] = -w2;
+  A[1][5] = -ssh;
+  b[1] = f.tsold * f.chsx * rw - (D(1) - ws) * w2 * ts - ssh * (qs - ws * dqs * ts) + frads -
+         f.pfluxs;
+  w2 = sl * cp;
+  A[2][2] = f.chlx * rw + wl * w2 + wl * slh * dql;
+  A[2][4] = -w2;
+  A[2][6] = -slh;
+  b[2] = f.tlold * f.chlx * rw - (D(1) - wl) * w2 * tl - slh * (ql - wl * dql * tl) + fradl -
+         f.pfluxl;
+  const D ru = xu * su, rs = xs * ss;
+  A[3][0] = -wu * ru;
+  A[3][1] = -ws * rs;
+  A[3][3] = cu + cl + ru + rs;
+  A[3][4] = -cl;
+  b[3] = cu * ta * f.tfac + (D(1) - wu) * ru * tu + (D(1) - ws) * rs * ts;
+  const D rl = xl * sl, ri = fi * si;
+  A[4][2] = -wl * rl;
+  A[4][3] = -cl;
+  A[4][4] = cl + rl + (D(1) - fi) * sg + ri;
+  A[4][7] = -wg * (D(1) - fi) * sg;
+  A[4][8] = -wi * ri;
+  b[4] = (D(1) - wl) * rl * tl + (D(1) - wg) * (D(1) - fi) * sg * tg + (D(1) - wi) * ri * ti;
+  const D ruq = xu * (suw + sut), rsq = xs * ssw;
+  A[5][0] = -wu * ruq * dqu;
+  A[5][1] = -ws * rsq * dqs;
+  A[5][5] = cu + cl + ruq + rsq;
+  A[5][6] = -cl;
+  b[5] = cu * f.qa + ruq * (qu - wu * dqu * tu) + rsq * (qs - ws * dqs * ts);
+  const D rlq = xl * (slw + slt), rgq = (D(1) - fi) * sg;
+  A[6][2] = -wl * rlq * dql;
+  A[6][5] = -cl;
+  A[6][6] = cl + rlq + rgq + fi * si;
+  A[6][7] = -wg * rgq * qgfac * dqg;
+  A[6][8] = -wi * fi * si * dqi;
+  b[6] = rlq * (ql - wl * dql * tl) + rgq * qgfac * (qg - wg * dqg * tg) +
+         fi * si * (qi - wi * dqi * ti);
+  D a1 = sg * cp, a2 = sg * f.hvasug;
+  A[7][4] = -a1;
+  A[7][6] = -a2;
+  A[7][7] = a1 + a2 * qgfac * dqg + f.cog + f.zirg;
+  b[7] = -a2 * qgfac * (qg - dqg * tg) + f.cog * f.tsoi1 + f.solg + f.firg + f.zirg * f.tgold;
+  a1 = si * cp;
+  a2 = si * f.hvasui;
+  A[8][4] = -a1;
+  A[8][6] = -a2;
+  A[8][8] = a1 + a2 * dqi + f.coi + f.ziri;
+  b[8] = -a2 * (qi - dqi * ti) + f.coi * f.tsno1 + f.soli + f.firi + f.ziri * f.tiold;
+  // bare-cell identity rows (canopy_exchange.py:709-747)
+  if ((xu + xs + xl) < eps) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+      for (int j = 0; j < 9; ++j) A[i][j] = (i == j) ? D(1) : D(0);
+    const D tt = ta * f.tfac;
+    b[0] = b[1] = b[2] = b[3] = b[4] = tt;
+    b[5] = b[6] = f.qa;
+    b[7] = b[7] + ((fradu - f.pfluxu) * xu + (frads - f.pfluxs) * xs + (fradl - f.pfluxl) * xl);
+  }
+  IBIS_TRACE_SYSTEM(A, b);
+  D x[9];
+  gauss9(A, b, x);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) o.t[j] = x[j];
+  o.fsena = cp * cu * (ta * f.tfac - x[3]);
+  o.fsenu = cp * su * (wu * x[0] + (D(1) - wu) * tu - x[3]);
+  o.fsens = cp * ss * (ws * x[1] + (D(1) - ws) * ts - x[3]);
+  o.fsenl = cp * sl * (wl * x[2] + (D(1) - wl) * tl - x[3]);
+  o.fseng = cp * sg * (wg * x[7] + (D(1) - wg) * tg - x[4]);
+  o.fseni = cp * si * (wi * x[8] + (D(1) - wi) * ti - x[4]);
+  o.fvapa = cu * (f.qa - x[5]);
+  const D quav = qu + wu * dqu * (x[0] - tu);
+  o.fvapuw = suw * (quav - x[5]);
+  o.fvaput = mx(D(0), sut * (quav - x[5]));
+  const D qsav = qs + ws * dqs * (x[1] - ts);
+  o.fvaps = ssw * (qsav - x[5]);
+  const D qlav = ql + wl * dql * (x[2] - tl);
+  o.fvaplw = slw * (qlav - x[6]);
+  o.fvaplt = mx(D(0), slt0 * (qlav - x[6]));
+  const D qgav = qg + wg * dqg * (x[7] - tg);
+  o.fvapg = sg * (qgfac * qgav - x[6]);
+  const D qiav = qi + wi * dqi * (x[8] - ti);
+  o.fvapi = si * (qiav - x[6]);
+  o.firg = f.firg - wg * f.zirg * (x[7] - f.tgold);
+  o.firi = f.firi - wi * f.ziri * (x[8] - f.tiold);
+  o.firb = f.firb + (D(1) - fi) * wg * f.zirg * (x[7] - f.tgold) +
+           fi * wi * f.ziri * (x[8] - f.tiold);
+  o.ginvap = o.fvapuw * xu + o.fvaps * xs + o.fvaplw * xl;
+  o.gsuvap = o.fvapg * (D(1) - fi) + o.fvapi * fi;
+  o.gtrans = o.fvaput * D(2) * f.lai1 * f.fu + o.fvaplt * D(2) * f.lai0 * f.fl * (D(1) - fi);
+}
+
+// ---------------------------------------------------------------------------------------
+// Tridiagonal solve (solvers.py:17-83, F77 tridia math.f:189)
+// ---------------------------------------------------------------------------------------
+template <class D, int K>
+IBIS_HD IBIS_INLINE void tridia(const D a[K], const D b[K], const D c[K], const D r[K], D x[K]) {
+  D gam[K];
+  D alpha = D(1) / b[0];
+  gam[0] = c[0] * alpha;
+  x[0] = r[0] * alpha;
+#pragma unroll
+  for (int j = 1; j < K - 1; ++j) {
+    alpha = D(1) / (b[j] - a[j] * gam[j - 1]);
+    gam[j] = c[j] * alpha;
+    x[j] = (r[j] - a[j] * x[j - 1]) * alpha;
+  }
+  x[K - 1] = (r[K - 1] - a[K - 1] * x[K - 2]) / (b[K - 1] - a[K - 1] * gam[K - 2]);
+#pragma unroll
+  for (int j = K - 2; j >= 0; --j) x[j] = x[j] - gam[j] * x[j + 1];
+}
+
+}  // namespace ibis

@@ -1,0 +1,922 @@
+// The hourly step for one cell: timestep_body (python-port/ibis/timestepping/loops.py:234-1284)
+// over the plane-major arena.  S = storage type, R = bulk arithmetic, D = implicit solves.
+#pragma once
+
+#include "ibis_physics_b.hpp"
+
+namespace ibis {
+
+// Host-computed constants shared by every cell of a launch.
+struct StepConst {
+  double dtime, tfac, zweight, nytimes_new;
+  int bedrock;
+};
+
+template <class S> struct Cell {
+  const S* in;
+  S* out;
+  int64_t pitch;
+  int64_t i;
+  IBIS_HD IBIS_INLINE S ld(int plane) const { return in[(int64_t)plane * pitch + i]; }
+  IBIS_HD IBIS_INLINE void st(int plane, S v) const { out[(int64_t)plane * pitch + i] = v; }
+};
+
+// Per-cell trace points for host-side debugging builds; empty in every product build.
+#ifndef IBIS_TRACE
+#define IBIS_TRACE(cell, tag, value) ((void)0)
+#endif
+
+#define LD(P) R(c.ld(P))
+#define ST(P, v) c.st((P), S(v))
+
+template <class S, class R, class D>
+IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisParams& P,
+                       const StepConst& K) {
+  IBIS_TRACE(c.i, "cell", 0);
+  const R eps = R(P.epsilon), tmelt = R(P.tmelt);
+  const R dtime = R(K.dtime);
+  const D dtD = D(K.dtime), epsD = D(P.epsilon), tmeltD = D(P.tmelt);
+  const D ch2oD = D(P.ch2o), ciceD = D(P.cice), rhowD = D(P.rhow), hfusD = D(P.hfus);
+  constexpr int NS = IBIS_NSOILAY;
+
+  // ---- inputs --------------------------------------------------------------------------
+  R tsoi[NS], wsoi[NS], wisoi[NS], poros[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    tsoi[k] = LD(IBIS_P_SOIL_TSOI + k);
+    wsoi[k] = LD(IBIS_P_SOIL_WSOI + k);
+    wisoi[k] = LD(IBIS_P_SOIL_WISOI + k);
+    poros[k] = LD(IBIS_P_SOIL_POROS + k);
+  }
+  const R tg0 = LD(IBIS_P_SOIL_TG), wpud = LD(IBIS_P_SOIL_WPUD), wipud = LD(IBIS_P_SOIL_WIPUD);
+  const R fi0 = LD(IBIS_P_SNOW_FI), ti0 = LD(IBIS_P_SNOW_TI), tlsub = LD(IBIS_P_SNOW_TLSUB);
+  R tsno[3], hsno[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    tsno[k] = LD(IBIS_P_SNOW_TSNO + k);
+    hsno[k] = LD(IBIS_P_SNOW_HSNO + k);
+  }
+  const R tu0 = LD(IBIS_P_CANOPY_TU), ts0 = LD(IBIS_P_CANOPY_TS), tl0 = LD(IBIS_P_CANOPY_TL);
+  const R ta = LD(IBIS_P_ATM_TA), qa = LD(IBIS_P_ATM_QA), ua = LD(IBIS_P_ATM_UA);
+  const R psurf = LD(IBIS_P_ATM_PSURF), coszen = LD(IBIS_P_ATM_COSZEN);
+  const R solad[2] = {LD(IBIS_P_ATM_SOLAD), LD(IBIS_P_ATM_SOLAD + 1)};
+  const R solai[2] = {LD(IBIS_P_ATM_SOLAI), LD(IBIS_P_ATM_SOLAI + 1)};
+  const R fira = LD(IBIS_P_ATM_FIRA), raina = LD(IBIS_P_ATM_RAINA), snowa = LD(IBIS_P_ATM_SNOWA);
+  const R lai[2] = {LD(IBIS_P_VEGETATION_LAI), LD(IBIS_P_VEGETATION_LAI + 1)};
+  const R sai[2] = {LD(IBIS_P_VEGETATION_SAI), LD(IBIS_P_VEGETATION_SAI + 1)};
+  const R fu = LD(IBIS_P_VEGETATION_FU), fl = LD(IBIS_P_VEGETATION_FL);
+  R w0[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w0[k] = LD(IBIS_P_CANOPY_WLIQU + k);
+
+  // ---- 1. setsoi (soil_physics.py:37-140) ------------------------------------------------
+  R consoi[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const R fsand = R(0.01) * LD(IBIS_P_SOIL_SAND + k);
+    const R fclay = R(0.01) * LD(IBIS_P_SOIL_CLAY + k);
+    const R fsilt = R(1) - fsand - fclay;
+    const R powliq = poros[k] * wsoi[k] * (R(1) - wisoi[k]);
+    const R powice = poros[k] * wisoi[k];
+    const R zcondry = fsand * R(0.300) + fsilt * R(0.265) + fclay * R(0.250);
+    consoi[k] = zcondry * r_pow(R(56), powliq) * r_pow(R(224), powice);
+    ST(IBIS_P_SOIL_CONSOI + k, consoi[k]);
+  }
+  R q1, q2, q3, q4, hvasug, hvasui;
+  {
+    const R pud = wpud + wipud;
+    const R zwpmax = R(P.zwpmax);
+    const R zwpud = clip(zwpmax * pud / mx(R(P.wpudmax), eps), R(0), zwpmax);
+    const R w1 = wsoi[0], wi1 = wisoi[0];
+    const R zwsoi = (R(1) - wi1) * w1 + wi1;
+    const bool soilw = zwsoi >= eps, hasp = zwpud >= eps;
+    const R r1 = R(1) / mx(zwsoi, eps);
+    const R r2 = R(1) / mx(pud, eps);
+    const bool above = tsoi[0] >= tmelt - R(1.0e-9);
+    q1 = soilw ? (hasp ? (R(1) - zwpud) * (R(1) - wi1) * w1 * r1 : (R(1) - wi1) * w1 * r1) : R(0);
+    q2 = soilw ? (hasp ? (R(1) - zwpud) * wi1 * r1 : wi1 * r1) : R(0);
+    q3 = hasp ? zwpud * wpud * r2 : (soilw ? R(0) : (above ? R(1) : R(0)));
+    q4 = hasp ? zwpud * wipud * r2 : (soilw ? R(0) : (above ? R(0) : R(1)));
+    const R zvap = R(P.hvap) + R(P.cvap) * (ta - tmelt) - R(P.ch2o) * (tsoi[0] - tmelt);
+    const R zsub = R(P.hsub) + R(P.cvap) * (ta - tmelt) - R(P.cice) * (tsoi[0] - tmelt);
+    hvasug = (q1 + q3) * zvap + (q2 + q4) * zsub;
+    hvasui = R(P.hsub) + R(P.cvap) * (ta - tmelt) - R(P.cice) * (tsno[0] - tmelt);
+    ST(IBIS_P_SOIL_QGLIF + 0, q1);
+    ST(IBIS_P_SOIL_QGLIF + 1, q2);
+    ST(IBIS_P_SOIL_QGLIF + 2, q3);
+    ST(IBIS_P_SOIL_QGLIF + 3, q4);
+    ST(IBIS_P_SOIL_HVASUG, hvasug);
+    ST(IBIS_P_SNOW_HVASUI, hvasui);
+  }
+
+  // ---- 2-5. wet fractions, radiation, cascade ----------------------------------------------
+  const Fwet<R> fw = fwetcal(tu0, ts0, tl0, w0, P);
+  RadOut<R> rad;
+  radiation(rad, coszen, solad, solai, lai, sai, fu, fl, tu0, ts0, tl0, fi0, tsno[0], wsoi[0],
+            wisoi[0], LD(IBIS_P_SOIL_ALBSAV), LD(IBIS_P_SOIL_ALBSAN), fw, P);
+  const D laiD[2] = {D(lai[0]), D(lai[1])}, saiD[2] = {D(sai[0]), D(sai[1])};
+  IrOut<D> ir;
+  irrad(ir, D(fira), laiD, saiD, D(fu), D(fl), D(fi0), D(tu0), D(ts0), D(tl0), D(tg0), D(ti0), P);
+  ST(IBIS_P_CANOPY_SOLU, rad.solu);
+  ST(IBIS_P_CANOPY_SOLS, rad.sols);
+  ST(IBIS_P_CANOPY_SOLL, rad.soll);
+  ST(IBIS_P_CANOPY_SOLG, rad.solg);
+  ST(IBIS_P_CANOPY_SOLI, rad.soli);
+  ST(IBIS_P_CANOPY_FIRU, ir.firu);
+  ST(IBIS_P_CANOPY_FIRS, ir.firs);
+  ST(IBIS_P_CANOPY_FIRL, ir.firl);
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    ST(IBIS_P_CANOPY_ASURD + b, rad.asurd[b]);
+    ST(IBIS_P_CANOPY_ASURI + b, rad.asuri[b]);
+  }
+  ST(IBIS_P_CANOPY_TOPPARU, rad.topparu);
+  ST(IBIS_P_CANOPY_TOPPARL, rad.topparl);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ST(IBIS_P_CANOPY_SCALCOEFU + j, rad.scalcoefu[j]);
+    ST(IBIS_P_CANOPY_SCALCOEFL + j, rad.scalcoefl[j]);
+  }
+  const R t12_0 = LD(IBIS_P_CANOPY_T12), t34_0 = LD(IBIS_P_CANOPY_T34);
+  Cascade<R> cas;
+  cascade(cas, tu0, ts0, tl0, t12_0, t34_0, w0, lai, sai, fu, fl, ta, qa, psurf, raina, snowa,
+          dtime, P);
+  const Fwet<R> fw2 = fwetcal(tu0, ts0, tl0, cas.w, P);
+  ST(IBIS_P_CANOPY_FWETU, fw2.fwetu);
+  ST(IBIS_P_CANOPY_FWETS, fw2.fwets);
+  ST(IBIS_P_CANOPY_FWETL, fw2.fwetl);
+  ST(IBIS_P_CANOPY_RLIQU, fw2.rliqu);
+  ST(IBIS_P_CANOPY_RLIQS, fw2.rliqs);
+  ST(IBIS_P_CANOPY_RLIQL, fw2.rliql);
+
+  // ---- 6. canopy exchange ------------------------------------------------------------------
+  // aerodynamics in the solve precision: turcof's profile ratios of exponentials cancel
+  Aero<D> aero;
+  const R ztop[2] = {LD(IBIS_P_VEGETATION_ZTOP), LD(IBIS_P_VEGETATION_ZTOP + 1)};
+  const R zbot[2] = {LD(IBIS_P_VEGETATION_ZBOT), LD(IBIS_P_VEGETATION_ZBOT + 1)};
+  {
+    const D ztD[2] = {D(ztop[0]), D(ztop[1])}, zbD[2] = {D(zbot[0]), D(zbot[1])};
+    canini(aero, D(ta), D(qa), D(psurf), D(ua), D(fi0), D(LD(IBIS_P_SOIL_Z0SOI)), D(fl), D(fu),
+           laiD, saiD, ztD, zbD, P);
+  }
+  ST(IBIS_P_CANOPY_RHOA, aero.rhoa);
+
+  // drystress (stomata.py:515-562)
+  R stressl[NS], stressu[NS], stresstl, stresstu;
+  {
+    const R sf = R(P.stressfac);
+    const R znorm = R(1) - r_exp(sf);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const R swilt = LD(IBIS_P_SOIL_SWILT + k), sfield = LD(IBIS_P_SOIL_SFIELD + k);
+      const R awc = clip((wsoi[k] * (R(1) - wisoi[k]) - swilt) / mx(sfield - swilt, R(1e-10)),
+                         R(0), R(1));
+      const R zwilt = clip((R(1) - r_exp(sf * awc)) / znorm, R(0), R(1));
+      stressl[k] = zwilt * R(P.froot[k][0]);
+      stressu[k] = zwilt * R(P.froot[k][1]);
+      ST(IBIS_P_SOIL_STRESSL + k, stressl[k]);
+      ST(IBIS_P_SOIL_STRESSU + k, stressu[k]);
+    }
+    stresstl = stressl[0];
+    stresstu = stressu[0];
+#pragma unroll
+    for (int k = 1; k < NS; ++k) {
+      stresstl = stresstl + stressl[k];
+      stresstu = stresstu + stressu[k];
+    }
+    ST(IBIS_P_SOIL_STRESSTL, stresstl);
+    ST(IBIS_P_SOIL_STRESSTU, stresstu);
+  }
+
+  // solid phase (loops.py:190-215) and the intermediates of loops.py:147-183
+  R csoi[NS], rhosoi[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const R cs_ = LD(IBIS_P_SOIL_CSOI + k), rh_ = LD(IBIS_P_SOIL_RHOSOI + k);
+    csoi[k] = cs_ > R(0) ? cs_ : R(P.csoi_mineral_val);
+    rhosoi[k] = rh_ > R(0) ? rh_ : R(P.rhosoi_val);
+  }
+  const R hsoi0 = R(P.hsoi[0]);
+  const R xu = R(2) * lai[1] * fu;
+  const R xs = R(2) * sai[1] * fu;
+  const R xl = R(2) * (lai[0] + sai[0]) * fl * (R(1) - fi0);
+
+  TvFix<D> F;
+  {
+    const R cog = consoi[0] / mx(R(0.5) * hsoi0, eps);
+    const R coi = R(P.consno) / mx(R(0.5) * mx(hsno[0], R(P.hsnotop)), eps);
+    const R zirg = R(4.0 * 0.95) * R(P.stef) * cube(tg0);
+    const R ziri = R(4.0 * 0.95) * R(P.stef) * cube(ti0);
+    F.tuold = tu0; F.tsold = ts0; F.tlold = tl0; F.tgold = tg0; F.tiold = ti0;
+    F.cp = aero.cp; F.tfac = D(K.tfac); F.xu = xu; F.xs = xs; F.xl = xl; F.fi = fi0;
+    F.fu = fu; F.fl = fl; F.lai0 = lai[0]; F.lai1 = lai[1]; F.sai0 = sai[0]; F.sai1 = sai[1];
+#pragma unroll
+    for (int j = 0; j < IBIS_NPFT; ++j) F.frac[j] = D(LD(IBIS_P_VEGETATION_FRAC + j));
+    F.chux = R(P.chu) + R(P.ch2o) * cas.w[0] + R(P.cice) * cas.w[3];
+    F.chsx = R(P.chs) + R(P.ch2o) * cas.w[1] + R(P.cice) * cas.w[4];
+    F.chlx = R(P.chl) + R(P.ch2o) * cas.w[2] + R(P.cice) * cas.w[5];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) F.w[j] = cas.w[j];
+    F.rliqu = fw2.rliqu; F.rliqs = fw2.rliqs; F.rliql = fw2.rliql;
+    F.fwetu = fw2.fwetu; F.fwets = fw2.fwets; F.fwetl = fw2.fwetl;
+    F.cog = cog; F.coi = coi; F.zirg = zirg; F.ziri = ziri;
+    F.wpud = wpud; F.wipud = wipud; F.wisoi1 = wisoi[0]; F.wsoi1 = wsoi[0];
+    F.swilt1 = LD(IBIS_P_SOIL_SWILT); F.suction1 = LD(IBIS_P_SOIL_SUCTION);
+    F.bex1 = LD(IBIS_P_SOIL_BEX); F.hvasug = hvasug; F.hvasui = hvasui;
+    F.tsoi1 = tsoi[0]; F.tsno1 = tsno[0];
+    F.solu = rad.solu; F.sols = rad.sols; F.soll = rad.soll; F.solg = rad.solg;
+    F.soli = rad.soli; F.firu = ir.firu; F.firs = ir.firs; F.firl = ir.firl;
+    F.firg = ir.firg; F.firi = ir.firi; F.firb = ir.firb;
+    F.pfluxu = cas.pfluxu; F.pfluxs = cas.pfluxs; F.pfluxl = cas.pfluxl;
+    F.ta = ta; F.qa = qa; F.psurf = psurf;
+  }
+  // ground heat capacity and water for the ground implicit weight (loops.py:164-181)
+  const R rwp = poros[0] * R(P.rhow);
+  const R chgx = R(P.ch2o) * wpud + R(P.cice) * wipud +
+                 ((R(1) - poros[0]) * csoi[0] * rhosoi[0] +
+                  rwp * (R(1) - wisoi[0]) * wsoi[0] * R(P.ch2o) + rwp * wisoi[0] * R(P.cice)) * hsoi0;
+  const R wlgx = wpud + rwp * (R(1) - wisoi[0]) * wsoi[0] * hsoi0;
+  const R wigx = wipud + rwp * wisoi[0] * hsoi0;
+
+  // Vmax of the big leaves by existence priority (stomata.py:598-650)
+  R ex[5];
+  ex[0] = LD(IBIS_P_VEGETATION_EXIST + 0);
+  ex[1] = LD(IBIS_P_VEGETATION_EXIST + 2);
+  ex[2] = LD(IBIS_P_VEGETATION_EXIST + 3);
+  const R vub = ex[0] > R(0.5) ? vrow[0] : (ex[1] > R(0.5) ? vrow[2] : vrow[4]);
+  const R vuc = ex[2] > R(0.5) ? vrow[3] : vrow[5];
+
+  Stom<R> sm;
+  sm.ci[0] = LD(IBIS_P_CANOPY_CI_UPPER); sm.ci[1] = LD(IBIS_P_CANOPY_CI_UPPER + 1);
+  sm.gs[0] = LD(IBIS_P_CANOPY_GS_UPPER); sm.gs[1] = LD(IBIS_P_CANOPY_GS_UPPER + 1);
+  sm.cs[0] = LD(IBIS_P_CANOPY_CS_UPPER); sm.cs[1] = LD(IBIS_P_CANOPY_CS_UPPER + 1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    sm.ci[2 + k] = LD(IBIS_P_CANOPY_CI_LOWER + k);
+    sm.gs[2 + k] = LD(IBIS_P_CANOPY_GS_LOWER + k);
+    sm.cs[2 + k] = LD(IBIS_P_CANOPY_CS_LOWER + k);
+  }
+  sm.a10[0] = LD(IBIS_P_CANOPY_A10SCALPARAMU);
+  sm.a10[1] = LD(IBIS_P_CANOPY_A10DAYLIGHTU);
+  sm.a10[2] = LD(IBIS_P_CANOPY_A10SCALPARAML);
+  sm.a10[3] = LD(IBIS_P_CANOPY_A10DAYLIGHTL);
+
+  // The canopy iterate (tu ts tl t12 t34 q12 q34 tg ti) and the implicit weights stay in the
+  // solve precision across the three passes; turcof and stomata read bulk-precision copies.
+  D cur[9] = {D(tu0), D(ts0), D(tl0), D(t12_0), D(t34_0), D(LD(IBIS_P_CANOPY_Q12)),
+              D(LD(IBIS_P_CANOPY_Q34)), D(tg0), D(ti0)};
+  D u2c = aero.u2, aloguc = aero.alogu, aloglc = aero.alogl;
+  D straml = D(1), strahl = D(1), stramu = D(1), strahu = D(1);
+  const D taD = ta, qaD = qa, uaD = ua;
+  D wp[5] = {D(1), D(1), D(1), D(1), D(1)};
+  TvOut<D> tv;
+  const R co2 = R(P.co2conc), zw = R(K.zweight);
+  const D chgxD = chgx, wlgxD = wlgx, wigxD = wigx;
+  for (int it = 0; it < 3; ++it) {
+    const bool first = it == 0;
+    Turcof<D> tc;
+    turcof(tc, aero, D(K.tfac), taD, qaD, uaD, cur[3], cur[4], cur[5], cur[6], dtD, first, u2c,
+           aloguc, aloglc, straml, strahl, stramu, strahu, P);
+    stomata(sm, R(cur[0]), R(cur[2]), R(cur[3]), R(cur[4]), R(cur[5]), R(cur[6]), R(tc.su), R(tc.sl),
+            psurf, rad, fw2.fwetu, fw2.fwetl, lai, sai, vub, vuc, vrow[8], vrow[11], vrow[10],
+            stresstu, stresstl, co2, zw, P);
+    if (!first) {
+      wp[0] = impexp(cur[0], F.chux, F.w[0], F.w[3], wp[0], P);
+      wp[1] = impexp(cur[1], F.chsx, F.w[1], F.w[4], wp[1], P);
+      wp[2] = impexp(cur[2], F.chlx, F.w[2], F.w[5], wp[2], P);
+      wp[3] = impexp(cur[7], chgxD, wlgxD, wigxD, wp[3], P);
+      wp[4] = impexp2(cur[8], F.tiold, wp[4], P);
+    }
+    D tcD[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) tcD[j] = sm.tcond[j];
+    turvap(tv, F, cur, tc.cu, tc.cl, tc.su, tc.ss, tc.sl, tc.sg, tc.si, tcD, wp[0], wp[1], wp[2], wp[3], wp[4], dtD, P);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cur[j] = tv.t[j];
+    IBIS_TRACE(c.i, "si", tc.si);
+    IBIS_TRACE(c.i, "wi", wp[4]);
+    IBIS_TRACE(c.i, "ti", cur[8]);
+  }
+  // ti clamp after the last pass (loops.py:604-611)
+  const R ti = R(mn(cur[8], tmeltD));
+  R tg = R(cur[7]);
+
+  // canopy outputs of the loop
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    ST(IBIS_P_CANOPY_CI_UPPER + g, sm.ci[g]);
+    ST(IBIS_P_CANOPY_GS_UPPER + g, sm.gs[g]);
+    ST(IBIS_P_CANOPY_CS_UPPER + g, sm.cs[g]);
+    ST(IBIS_P_CANOPY_AN_UPPER + g, sm.an[g]);
+    ST(IBIS_P_CANOPY_AG_UPPER + g, sm.ag[g]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ST(IBIS_P_CANOPY_CI_LOWER + k, sm.ci[2 + k]);
+    ST(IBIS_P_CANOPY_GS_LOWER + k, sm.gs[2 + k]);
+    ST(IBIS_P_CANOPY_CS_LOWER + k, sm.cs[2 + k]);
+    ST(IBIS_P_CANOPY_AN_LOWER + k, sm.an[2 + k]);
+    ST(IBIS_P_CANOPY_AG_LOWER + k, sm.ag[2 + k]);
+  }
+  ST(IBIS_P_CANOPY_A10SCALPARAMU, sm.a10[0]);
+  ST(IBIS_P_CANOPY_A10DAYLIGHTU, sm.a10[1]);
+  ST(IBIS_P_CANOPY_A10SCALPARAML, sm.a10[2]);
+  ST(IBIS_P_CANOPY_A10DAYLIGHTL, sm.a10[3]);
+  ST(IBIS_P_CANOPY_T12, cur[3]);
+  ST(IBIS_P_CANOPY_T34, cur[4]);
+  ST(IBIS_P_CANOPY_Q12, cur[5]);
+  ST(IBIS_P_CANOPY_Q34, cur[6]);
+  ST(IBIS_P_CANOPY_FSENU, tv.fsenu);
+  ST(IBIS_P_CANOPY_FSENS, tv.fsens);
+  ST(IBIS_P_CANOPY_FSENL, tv.fsenl);
+  ST(IBIS_P_CANOPY_FSENG, tv.fseng);
+  ST(IBIS_P_CANOPY_FSENI, tv.fseni);
+  ST(IBIS_P_CANOPY_FVAPG, tv.fvapg);
+  ST(IBIS_P_CANOPY_FVAPI, tv.fvapi);
+  ST(IBIS_P_CANOPY_FIRG, tv.firg);
+  ST(IBIS_P_CANOPY_FIRI, tv.firi);
+  ST(IBIS_P_CANOPY_FIRB, tv.firb);
+  ST(IBIS_P_CANOPY_RAING, cas.raing);
+  ST(IBIS_P_CANOPY_SNOWG, cas.snowg);
+  ST(IBIS_P_HYDROLOGY_GINVAP, tv.ginvap);
+  ST(IBIS_P_HYDROLOGY_GSUVAP, tv.gsuvap);
+  ST(IBIS_P_HYDROLOGY_GTRANS, tv.gtrans);
+  ST(IBIS_P_HYDROLOGY_RAINL, cas.rainl);
+  ST(IBIS_P_HYDROLOGY_SNOWL, cas.snowl);
+  ST(IBIS_P_HYDROLOGY_TRAINL, cas.trainl);
+  ST(IBIS_P_HYDROLOGY_TSNOWL, cas.tsnowl);
+  ST(IBIS_P_HYDROLOGY_WTOT, R(0));
+
+  // ---- 7. cascad2 (interception.py:423-463) -------------------------------------------------
+  R tu = R(cur[0]), ts = R(cur[1]), tl = R(cur[2]);
+  R w[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) w[j] = cas.w[j];
+  R fvapa = R(tv.fvapa), fsena = R(tv.fsena);
+  steph2o2(tu, w[0], w[3], fu, R(2) * lai[1], fw2.rliqu, R(tv.fvapuw), R(P.chu), dtime, ta, fvapa,
+           fsena, P);
+  steph2o2(ts, w[1], w[4], fu, R(2) * sai[1], fw2.rliqs, R(tv.fvaps), R(P.chs), dtime, ta, fvapa,
+           fsena, P);
+  steph2o2(tl, w[2], w[5], (R(1) - fi0) * fl, R(2) * (lai[0] + sai[0]), fw2.rliql, R(tv.fvaplw),
+           R(P.chl), dtime, ta, fvapa, fsena, P);
+  ST(IBIS_P_DIAG_FSENA, fsena);
+  ST(IBIS_P_DIAG_FVAPA, fvapa);
+
+  // ---- 8. noveg (interception.py:470-495) ---------------------------------------------------
+  {
+    const R tav = (R(1) - fi0) * tg + fi0 * ti;
+    if (lai[1] == R(0) || fu == R(0)) { tu = tav; w[0] = R(0); w[3] = R(0); }
+    if (sai[1] == R(0) || fu == R(0)) { ts = tav; w[1] = R(0); w[4] = R(0); }
+    const R x = R(2) * (lai[0] + sai[0]);
+    const R y = fl * (R(1) - fi0);
+    if (x == R(0) || y == R(0)) { tl = tav; w[2] = R(0); w[5] = R(0); }
+  }
+
+  // ---- 9. surface fluxes with the flux clamp (loops.py:665-674) ------------------------------
+  const R fm = R(P.flux_max);
+  const R heatg = rad.solg + clip(R(tv.firg), -fm, fm) - clip(R(tv.fseng), -fm, fm) -
+                  hvasug * R(tv.fvapg);
+  const R heati = rad.soli + clip(R(tv.firi), -fm, fm) - clip(R(tv.fseni), -fm, fm) -
+                  hvasui * R(tv.fvapi);
+  ST(IBIS_P_SNOW_HEATI, heati);
+
+  // ---- 10. snow (loops.py:679-848) ----------------------------------------------------------
+  const R rhos = R(P.rhos), cice = R(P.cice), ch2o = R(P.ch2o), hfus = R(P.hfus);
+  const R fhtop_snow = heati +
+                       cas.rainl * (ch2o * (cas.trainl - tmelt) + hfus + cice * (tmelt - tsno[0])) +
+                       cas.snowl * cice * (cas.tsnowl - tsno[0]);
+  ST(IBIS_P_HYDROLOGY_FHTOP_SNOW, fhtop_snow);
+  const R xlb = fl * R(2) * (lai[0] + sai[0]);
+  R hs[3] = {hsno[0], hsno[1], hsno[2]};
+  if (fi0 > R(0)) hs[0] = hsno[0] + (cas.rainl + cas.snowl - R(tv.fvapi)) * (dtime / rhos);
+  // snowheat (snow_physics.py:109-229), solved in D
+  R tsn[3];
+  R tlsub_h;
+  D sflo_bot;
+  {
+    const D hfake = D(0.01), conbur = D(2);
+    const D rs = D(0.5) / D(P.consno);
+    D hsD[3] = {hs[0], hs[1], hs[2]};
+    D con[5];
+    con[0] = D(0);
+    con[1] = D(1) / (mx(hsD[0], hfake) * rs + mx(hsD[1], hfake) * rs);
+    con[2] = D(1) / (mx(hsD[1], hfake) * rs + mx(hsD[2], hfake) * rs);
+    con[3] = D(1) / (mx(hsD[2], hfake) * rs + D(0.5) * D(xlb) / conbur);
+    con[4] = D(1) / (D(xlb) * (D(0.5) / conbur) + D(0.5) * D(P.hsoi[0]) / mx(D(consoi[0]), D(1e-12)));
+    D a[4], b[4], cc[4], r[4], x[4];
+    const D rdt = dtD / (D(P.rhos) * ciceD);
+    const D tsnD[3] = {tsno[0], tsno[1], tsno[2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const D dt = rdt / mx(hsD[k], hfake);
+      a[k] = -dt * con[k];
+      b[k] = D(1) + dt * (con[k] + con[k + 1]);
+      cc[k] = -dt * con[k + 1];
+      r[k] = tsnD[k] + dt * D(0);
+      if (k == 0) r[k] = r[k] + dt * D(fhtop_snow);
+    }
+    const D dti = D(xlb) * D(P.chl) / dtD;
+    a[3] = -con[3];
+    b[3] = dti + (con[3] + con[4]);
+    cc[3] = D(0);
+    r[3] = dti * D(tlsub) + D(0) + con[4] * (D(tsoi[0]) - D(0));
+    tridia<D, 4>(a, b, cc, r, x);
+    D sflo = D(fhtop_snow);
+    const D rh = D(P.rhos) * ciceD / dtD;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sflo = sflo - rh * hsD[k] * (x[k] - tsnD[k]);
+    sflo_bot = sflo - D(xlb) * (D(P.chl) / dtD) * (x[3] - D(tlsub));
+    const bool has = fi0 > R(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tsn[k] = has ? R(x[k]) : tsno[k];
+    tlsub_h = has ? R(x[3]) : tlsub;
+  }
+  ST(IBIS_P_HYDROLOGY_SFLO_BOT, sflo_bot);
+  // new snow on the snow-free area, mix, hinit
+  const R hsnomin = R(P.hsnomin), hsnotop = R(P.hsnotop);
+  R ht = hs[0] + hs[1] + hs[2];
+  const R hts = ht > R(0) ? ht : hsnomin;
+  R dfi = (R(1) - fi0) * dtime * cas.snowg / (rhos * hts);
+  dfi = mn(dfi, R(1) - fi0);
+  const R fi1 = fi0 + dfi;
+  if (fi1 > R(0)) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tsn[k] = (tsn[k] * fi0 + cas.tsnowg * dfi) / mx(fi1, eps);
+  }
+  if (fi0 < eps && dfi > R(0)) {
+    hs[0] = hsnotop;
+    hs[1] = hs[2] = (hsnomin - hsnotop) / R(2);
+  }
+  // snow_melt (snow_physics.py:236-256)
+  R smelt, zheat;
+  {
+    R zm[3], zh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const R xh = rhos * hs[k] * cice * mx(tsn[k] - tmelt, R(0));
+      const R xm = mn(rhos * hs[k], xh / hfus);
+      hs[k] = hs[k] - xm / rhos;
+      tsn[k] = mn(tsn[k], tmelt);
+      zm[k] = fi1 * xm;
+      zh[k] = fi1 * (xh - hfus * xm);
+    }
+    smelt = zm[0] + zm[1] + zm[2];
+    zheat = zh[0] + zh[1] + zh[2];
+  }
+  // snow_coverage (snow_physics.py:263-290): sqrt(a^2 + e) - a cancels for a thin new pack,
+  // so it runs in the solve precision
+  R fi2;
+  {
+    const D hsnominD = D(P.hsnomin);
+    const D hsnoruf = mn(D(0.70), mx(hsnominD + D(0.05), D(fl) * D(ztop[0])));
+    D htc = D(hs[0]) + D(hs[1]) + D(hs[2]);
+    htc = htc < D(1e-10) ? hsnominD : htc;
+    const D zhh = D(0.5) * hsnominD;
+    const D zdh = mx(hsnoruf - hsnominD, epsD);
+    D f2 = (-zhh + r_sqrt(zhh * zhh + zdh * D(fi1) * htc)) / zdh;
+    f2 = clip(f2, D(0), D(P.fimax));
+    const D scale = D(fi1) / mx(f2, epsD);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hs[k] = R(D(hs[k]) * scale);
+    fi2 = R(f2);
+  }
+  // vadapt (snow_physics.py:27-96)
+  if ((hs[0] + hs[1] + hs[2]) > eps) {
+    R zold[4];
+    zold[0] = R(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zold[k + 1] = zold[k] + hs[k];
+    const R htt = zold[3];
+    const bool thick = htt >= R(3) * hsnotop;
+    const R h1 = thick ? (htt - hsnotop) / R(2) : mx(htt / R(3), eps);
+    R hn[3] = {thick ? hsnotop : h1, h1, h1};
+    R zn[4];
+    zn[0] = R(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zn[k + 1] = zn[k] + hn[k];
+    R tn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      R zh = R(0);
+#pragma unroll
+      for (int ko = 0; ko < 3; ++ko) {
+        const R ov = mx(mn(zold[ko + 1], zn[k + 1]) - mx(zold[ko], zn[k]), R(0));
+        zh = zh + tsn[ko] * ov;
+      }
+      tn[k] = zh / mx(hn[k], eps);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { hs[k] = hn[k]; tsn[k] = tn[k]; }
+  }
+  // fimin melt-all
+  R fi_new = fi2;
+  if (fi2 < R(P.fimin)) {
+    const R mass = rhos * (hs[0] + hs[1] + hs[2]) * fi2;
+    R hq[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hq[k] = rhos * hs[k] * fi2 * (cice * (tmelt - tsn[k]) + hfus);
+    smelt = smelt + mass;
+    zheat = zheat + -(hq[0] + hq[1] + hq[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { hs[k] = R(0); tsn[k] = tmelt; }
+    fi_new = R(0);
+  }
+  // buried vegetation (loops.py:802-848)
+  const R dfc = fi_new - fi0;
+  R tlsub_o = tlsub_h;
+  if (dfc > R(0)) {
+    tlsub_o = (tlsub_h * fi0 + tl * dfc) / mx(fi_new, eps);
+    zheat = zheat + dfc * xlb * (w[2] * (ch2o * (tl - tmelt) + hfus + cice * (tmelt - tsn[2])) +
+                                 w[5] * cice * (tl - tsn[2]));
+    hs[2] = hs[2] + dfc * xlb * (w[2] + w[5]) / (rhos * mx(fi_new, eps));
+  }
+  if (dfc < R(0)) {
+    zheat = zheat - dfc * xlb * R(P.chl) * (tlsub_h - tl);
+    const R rdec = (R(1) - fi0) / mx(R(1) - fi_new, eps);
+    w[2] = w[2] * rdec;
+    w[5] = w[5] * rdec;
+  }
+  if (!(fi_new > R(0))) tlsub_o = tl;
+  ST(IBIS_P_SNOW_FI, fi_new);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ST(IBIS_P_SNOW_TSNO + k, tsn[k]);
+    ST(IBIS_P_SNOW_HSNO + k, hs[k]);
+  }
+  ST(IBIS_P_SNOW_TI, ti);
+  ST(IBIS_P_SNOW_TLSUB, tlsub_o);
+
+  // areal averages (loops.py:850-894)
+  const R rdt = R(1) / dtime;
+  const R heatg_avg = (R(1) - fi0) * heatg + fi0 * R(sflo_bot) + zheat * rdt;
+  ST(IBIS_P_SOIL_HEATG, heatg_avg);
+  const R fvapg_avg = clip((R(1) - fi0) * R(tv.fvapg), R(-1.0e-3), R(1.0e-3));
+  const R raing_nf = (R(1) - fi0) * cas.raing;
+  const R smelt_rate = smelt * rdt;
+  const R r_un = raing_nf + smelt_rate;
+  const bool rzero = r_un < eps;
+  const R raing_avg = rzero ? R(0) : r_un;
+  const R traing_avg = rzero ? R(0) : (raing_nf * cas.traing + smelt_rate * tmelt) / mx(r_un, eps);
+  ST(IBIS_P_HYDROLOGY_RAING_AVG, raing_avg);
+  ST(IBIS_P_HYDROLOGY_TRAING_AVG, traing_avg);
+
+  // ---- 11. soil (loops.py:896-1125), in D ----------------------------------------------------
+  {
+    D hso[NS + IBIS_NBEDROCK];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) hso[k] = D(P.hsoi[k]);
+#pragma unroll
+    for (int k = 0; k < IBIS_NBEDROCK; ++k) hso[NS + k] = D(P.bed_hsoi[k]);
+    D wsD[NS], wiD[NS], poD[NS], tsD[NS], csD[NS], rhD[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      wsD[k] = wsoi[k]; wiD[k] = wisoi[k]; poD[k] = poros[k]; tsD[k] = tsoi[k];
+      csD[k] = csoi[k]; rhD[k] = rhosoi[k];
+    }
+    const D wpudmax = D(P.wpudmax);
+    D wpD = wpud, wipD = wipud;
+    D grunof = mn(wpD, mx(D(0), wpD + wipD - wpudmax)) / dtD;
+    wpD = wpD - grunof * dtD;
+    const D zfrez = clip((tsD[0] - tmeltD + D(0.5)) * D(2), D(0), D(1));
+    const D zrunf = zfrez * clip((wpD + wipD) / wpudmax, D(0), D(1));
+    const D rav = D(raing_avg);
+    wpD = wpD + (D(1) - zrunf) * rav * dtD;
+    grunof = grunof + zrunf * rav;
+    const D fvg = D(fvapg_avg);
+    const bool evap = fvg >= D(0);
+    const D edt = fvg * dtD;
+    const D qD[4] = {q1, q2, q3, q4};
+    const D fwtop = evap ? -qD[0] * fvg : D(0);
+    D wisoi0n = wiD[0], wsoi0a = wsD[0];
+    if (evap) {
+      wpD = wpD - qD[2] * edt;
+      wipD = wipD - qD[3] * edt;
+      const D isub = qD[1] * edt / mx(rhowD * poD[0] * hso[0], epsD);
+      wisoi0n = mx(D(0), wiD[0] - isub);
+      wsoi0a = ((D(1) - wisoi0n) > epsD) ? wsD[0] * (D(1) - wiD[0]) / (D(1) - wisoi0n) : wsD[0];
+    } else {
+      wpD = wpD - (qD[0] + qD[2]) * edt;
+      wipD = wipD - (qD[1] + qD[3]) * edt;
+    }
+    wpD = mx(wpD, D(0));
+    wipD = mx(wipD, D(0));
+    const D zdp = mx(D(1) - wisoi0n, D(0));
+    const D zdpud = rhowD * dtD * (zdp * zdp) * D(LD(IBIS_P_SOIL_HYDRAUL));
+    const D fwpud = mx(D(0), mn(wpD, zdpud)) / dtD;
+    D c0pud0 = ch2oD * wpD + ciceD * wipD;
+    if (evap) { wsD[0] = wsoi0a; wiD[0] = wisoi0n; }
+    D porosflo[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) porosflo[k] = mx(poD[k] * mx(D(1) - wiD[k], epsD), epsD);
+
+    // transpiration extraction per layer (loops.py:990-1048)
+    const D gtransu = D(tv.fvaput) * D(2) * D(lai[1]) * D(fu);
+    const D gtransl = D(tv.fvaplt) * D(2) * D(lai[0]) * D(fl) * (D(1) - D(fi0));
+    ST(IBIS_P_HYDROLOGY_GTRANSU, gtransu);
+    ST(IBIS_P_HYDROLOGY_GTRANSL, gtransl);
+    const D sstu = mx(D(stresstu), epsD), sstl = mx(D(stresstl), epsD);
+    const D tuD = tu, tlD = tl;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const D wu_ = (D(stresstu) >= D(1)) ? D(P.froot[k][1]) : D(stressu[k]) / sstu;
+      const D wl_ = (D(stresstl) >= D(1)) ? D(P.froot[k][0]) : D(stressl[k]) / sstl;
+      const D upu = gtransu * wu_;
+      const D upl = gtransl * wl_;
+      wsD[k] = wsD[k] + (-dtD * (upu + upl) / (rhowD * porosflo[k] * hso[k]));
+      wsD[k] = mx(wsD[k], D(0));
+      const D cpud = (k == 0) ? c0pud0 : D(0);
+      const D cx = cpud + ((D(1) - poD[k]) * csD[k] * rhD[k] +
+                           poD[k] * (D(1) - wiD[k]) * wsD[k] * ch2oD * rhowD +
+                           poD[k] * wiD[k] * ciceD * rhowD) * hso[k];
+      tsD[k] = tsD[k] + (-dtD * ch2oD * (upu * (tuD - tsD[k]) + upl * (tlD - tsD[k])) /
+                         mx(cx, epsD));
+    }
+    const D fhtop = D(heatg_avg) + rav * ch2oD * (D(traing_avg) - tsD[0]) -
+                    grunof * ch2oD * mx(tmeltD - tsD[0], D(0));
+    ST(IBIS_P_HYDROLOGY_FHTOP_SOIL, fhtop);
+    ST(IBIS_P_HYDROLOGY_GRUNOF, grunof);
+
+    // soilh2o (soil_physics.py:196-373)
+    D wn[NS], wflo[NS + 1], wpud_n;
+    {
+      D hyd[NS], suc[NS], bex[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        hyd[k] = D(LD(IBIS_P_SOIL_HYDRAUL + k));
+        suc[k] = D(LD(IBIS_P_SOIL_SUCTION + k));
+        bex[k] = D(LD(IBIS_P_SOIL_BEX + k));
+      }
+      D e[NS + 1], f[NS + 1], g[NS + 1];
+      e[0] = f[0] = g[0] = D(0);
+      f[NS] = g[NS] = D(0);
+      const D dmin = D(1e-9);
+#pragma unroll
+      for (int k = 1; k < NS; ++k) {
+        const D hm = D(0.5) * (hso[k - 1] + hso[k]);
+        const D weip = D(0.5) * hso[k] / hm;
+        const D weim = D(1) - weip;
+        const D a = weim * hyd[k - 1] + weip * hyd[k];
+        const D b = weim * hyd[k - 1] * suc[k - 1] * bex[k - 1] + weip * hyd[k] * suc[k] * bex[k];
+        const D zbex = weim * bex[k - 1] + weip * bex[k];
+        const int nr = (int)r_floor(zbex + D(0.5));
+        const int m = 2 * nr + 3, nn = nr + 2;
+        const D wm = P.h2o_consistent_weights ? weip * wsD[k - 1] + weim * wsD[k]
+                                              : weim * wsD[k - 1] + weip * wsD[k];
+        const D wa = mn(wm, D(1));
+        D bwn1 = b * ipow(wa, nn - 1);
+        D bwn = bwn1 * wa;
+        if (bwn < dmin) bwn1 = D(0);
+        bwn = mx(bwn, dmin);
+        const D dw = (wsD[k] - wsD[k - 1]) / hm;
+        const D ek = (D(-1) + D(m)) * a * ipow(wa, m) + (D(0) - D(nn) * bwn1 * wa) * dw;
+        const D fk = -D(m) * a * ipow(wa, m - 1) + D(nn) * bwn1 * dw;
+        D zi = mx(D(0), D(1) - mx(wiD[k - 1], wiD[k]));
+        zi = zi * zi;
+        e[k] = zi * ek;
+        f[k] = zi * fk;
+        g[k] = zi * bwn;
+      }
+      {
+        const D wb = mn(wsD[NS - 1], D(1));
+        const int mb = 2 * (int)r_floor(bex[NS - 1] + D(0.5)) + 3;
+        const D eb = -hyd[NS - 1] * ipow(wb, mb) * D(P.bperm);
+        D zi = mx(D(0), D(1) - wiD[NS - 1]);
+        e[NS] = (zi * zi) * eb;
+      }
+      D d1[NS], d2[NS], d3[NS], r[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const D dt = dtD / mx(porosflo[k] * hso[k], D(1e-30));
+        D hm = (k == 0) ? D(0.5) * hso[0] : D(0.5) * hso[k - 1] + D(0.5) * hso[k];
+        hm = mx(hm, D(1e-30));
+        const int km1 = k > 0 ? k - 1 : 0;
+        d1[k] = dt * (f[k] * D(0.5) * hso[k] / hm - g[k] / hm);
+        r[k] = wsD[k] + dt * (e[k + 1] - e[k]);
+        if (k == 0) {
+          r[k] = r[k] + dt * (fwtop + fwpud) / rhowD;
+          d1[k] = D(0);
+        }
+        if (k < NS - 1) {
+          const D hp = mx(D(0.5) * (hso[k] + hso[k + 1]), D(1e-30));
+          d2[k] = D(1) + dt * (-f[k + 1] * D(0.5) * hso[k + 1] / hp + f[k] * D(0.5) * hso[km1] / hm +
+                               g[k + 1] / hp + g[k] / hm);
+          d3[k] = dt * (-f[k + 1] * D(0.5) * hso[k] / hp - g[k + 1] / hp);
+        } else {
+          d2[k] = D(1) + dt * (-f[k + 1] + f[k] * D(0.5) * hso[km1] / hm + g[k] / hm);
+          d3[k] = D(0);
+        }
+      }
+      tridia<D, NS>(d1, d2, d3, r, wn);
+      wflo[NS] = -rhowD * e[NS];
+      D fsq = D(0);
+#pragma unroll
+      for (int k = NS - 1; k >= 0; --k) {
+        const D zz = rhowD * poD[k] * mx(epsD, D(1) - wiD[k]) * hso[k];
+        D wk = wn[k] + dtD * fsq / zz;
+        const D fsqn = mx(wk - D(1), D(0)) * zz / dtD;
+        wk = mn(wk, D(1));
+        wflo[k] = wflo[k + 1] + (wk - wsD[k]) * zz / dtD;
+        wn[k] = wk;
+        fsq = fsqn;
+      }
+      wpud_n = mx(wpD + (fsq - fwpud) * dtD, D(0));
+      wflo[0] = wflo[0] + fsq - fwpud;
+    }
+    ST(IBIS_P_HYDROLOGY_GDRAIN, wflo[NS]);
+    // c1pud / soilheat with bedrock (soil_physics.py:385-490, bedrock.py:88-153)
+    const D c1pud0 = ch2oD * wpud_n + ciceD * wipD;
+    constexpr int NT = NS + IBIS_NBEDROCK;
+    const int nt = K.bedrock ? NT : NS;
+    D tT[NT], otT[NT], conT[NT], csT[NT], rhT[NT], poT[NT], owT[NT], wT[NT], wiT[NT];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      tT[k] = tsD[k]; otT[k] = tsD[k]; conT[k] = D(consoi[k]); csT[k] = csD[k]; rhT[k] = rhD[k];
+      poT[k] = poD[k]; owT[k] = wsD[k]; wT[k] = wn[k]; wiT[k] = wiD[k];
+    }
+#pragma unroll
+    for (int k = NS; k < NT; ++k) {
+      const D td = K.bedrock ? D(LD(IBIS_P_SOIL_TSOI_DEEP + (k - NS))) : D(0);
+      tT[k] = td; otT[k] = td; conT[k] = D(P.bed_consoi); csT[k] = D(P.bed_csoi);
+      rhT[k] = D(P.bed_rhosoi); poT[k] = D(0); owT[k] = D(0); wT[k] = D(0); wiT[k] = D(0);
+    }
+    D tn[NT], hflo[NT + 1];
+    {
+      D con[NT + 1], wh[NT + 1];
+      con[0] = D(0);
+      wh[0] = wflo[0] * ch2oD * tT[0];
+#pragma unroll
+      for (int k = 1; k < NT; ++k) {
+        con[k] = D(1) / (D(0.5) * (hso[k - 1] / mx(conT[k - 1], D(1e-12)) +
+                                   hso[k] / mx(conT[k], D(1e-12))));
+        if (k < NS) {
+          const D tav = (hso[k] * tT[k - 1] + hso[k - 1] * tT[k]) / (hso[k - 1] + hso[k]);
+          wh[k] = wflo[k] * ch2oD * tav;
+        }
+      }
+      con[NT] = D(0);
+      const D drain = wflo[NS] * ch2oD * tT[NS - 1];
+#pragma unroll
+      for (int k = NS; k < NT + 1; ++k) wh[k] = drain;
+      if (!K.bedrock) con[NS] = D(0);
+      D d1[NT], d2[NT], d3[NT], r[NT], c0[NT], c1[NT];
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        const D r1 = (D(1) - poT[k]) * csT[k] * rhT[k];
+        const D r2 = poT[k] * (D(1) - wiT[k]) * ch2oD * rhowD;
+        const D r3 = poT[k] * wiT[k] * ciceD * rhowD;
+        c0[k] = (k == 0 ? c0pud0 : D(0)) + (r1 + r2 * owT[k] + r3) * hso[k];
+        c1[k] = (k == 0 ? c1pud0 : D(0)) + (r1 + r2 * wT[k] + r3) * hso[k];
+        const D rw = dtD / c1[k];
+        const D conb = (k + 1 < nt) ? con[k + 1] : D(0);
+        d1[k] = -rw * con[k];
+        d2[k] = D(1) + rw * (con[k] + conb);
+        d3[k] = -rw * conb;
+        r[k] = (c0[k] / c1[k]) * tT[k] + rw * (D(0) + D(0) + wh[k] - wh[k + 1]);
+        if (k == 0) r[k] = r[k] + (dtD / c1[k]) * fhtop;
+      }
+      if (K.bedrock) {
+        tridia<D, NT>(d1, d2, d3, r, tn);
+      } else {
+        tridia<D, NS>(d1, d2, d3, r, tn);
+#pragma unroll
+        for (int k = NS; k < NT; ++k) tn[k] = D(0);
+      }
+      hflo[0] = fhtop;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) hflo[k + 1] = hflo[k] - (c1[k] * tn[k] - c0[k] * otT[k]) / dtD;
+    }
+#pragma unroll
+    for (int k = 0; k <= NS; ++k) ST(IBIS_P_SOIL_HFLO + k, hflo[k]);
+    if (K.bedrock) {
+#pragma unroll
+      for (int k = 0; k < IBIS_NBEDROCK; ++k) ST(IBIS_P_SOIL_TSOI_DEEP + k, tn[NS + k]);
+    }
+    // wadjust (soil_physics.py:147-182)
+    D gadj = D(0);
+    D wa[NS], wia[NS];
+    {
+      D dsum = D(0);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const D t0 = hso[k] * poD[k] * rhowD * ((D(1) - wiD[k]) * wn[k] + wiD[k]);
+        D ww = mn(D(1), wn[k] < epsD ? D(0) : wn[k]);
+        const D wi = mn(D(1), wiD[k] < epsD ? D(0) : wiD[k]);
+        if (wi >= D(1) - epsD) ww = D(0);
+        const D t1 = hso[k] * poD[k] * rhowD * ((D(1) - wi) * ww + wi);
+        dsum = (k == 0) ? (t1 - t0) : dsum + (t1 - t0);
+        wa[k] = ww;
+        wia[k] = wi;
+      }
+      gadj = dsum / dtD;
+    }
+    // freeze_thaw (soil_physics.py:497-560)
+    D porosflo_f[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const D zp = mx(poD[k], epsD);
+      const D rw = (k == 0 ? c1pud0 : D(0)) / (rhowD * hso[k]) + (D(1) - zp) * csD[k] * rhD[k] / rhowD;
+      D chav = rw + zp * (D(1) - wia[k]) * wa[k] * ch2oD + zp * wia[k] * ciceD;
+      const D zwk = mn(D(1), wa[k]);
+      D dh = chav * (tmeltD - tn[k]);
+      D dw = mn(zp * (D(1) - wia[k]) * zwk, mx(D(0), dh / hfusD));
+      D wik = wia[k] + dw / zp;
+      D wk = wa[k] - (dw / zp) * (D(1) - zwk) / mx(epsD, D(1) - wik);
+      chav = rw + zp * (D(1) - wik) * wk * ch2oD + zp * wik * ciceD;
+      D tk = tmeltD - (dh - hfusD * dw) / chav;
+      dh = chav * (tk - tmeltD);
+      dw = mn(zp * wik, mx(D(0), dh / hfusD));
+      wik = wik - dw / zp;
+      wk = wk + (dw / zp) * (D(1) - wk) / mx(epsD, D(1) - wik);
+      chav = rw + zp * (D(1) - wik) * wk * ch2oD + zp * wik * ciceD;
+      tk = tmeltD + (dh - hfusD * dw) / chav;
+      tn[k] = tk;
+      wa[k] = wk;
+      wia[k] = wik;
+      porosflo_f[k] = poD[k] * mx(epsD, D(1) - wik);
+    }
+    // puddle_freeze_thaw (soil_physics.py:567-615)
+    D wpf, wipf;
+    {
+      const D rw = ((D(1) - poD[0]) * csD[0] * rhD[0] + poD[0] * (D(1) - wia[0]) * wa[0] * ch2oD * rhowD +
+                    poD[0] * wia[0] * ciceD * rhowD) * hso[0];
+      D wp_ = wpud_n, wip_ = wipD;
+      D chav = ch2oD * wp_ + ciceD * wip_ + rw;
+      D dh = chav * (tmeltD - tn[0]);
+      D dw = mn(wp_, mx(D(0), dh / hfusD));
+      wip_ = wip_ + dw;
+      wp_ = wp_ - dw;
+      chav = ch2oD * wp_ + ciceD * wip_ + rw;
+      D t0 = tmeltD - (dh - hfusD * dw) / chav;
+      dh = chav * (t0 - tmeltD);
+      dw = mn(wip_, mx(D(0), dh / hfusD));
+      wip_ = wip_ - dw;
+      wp_ = wp_ + dw;
+      chav = ch2oD * wp_ + ciceD * wip_ + rw;
+      t0 = tmeltD + (dh - hfusD * dw) / chav;
+      tn[0] = t0;
+      wpf = mx(wp_, D(0));
+      wipf = mx(wip_, D(0));
+    }
+    // second wadjust
+    {
+      D dsum = D(0);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const D t0 = hso[k] * poD[k] * rhowD * ((D(1) - wia[k]) * wa[k] + wia[k]);
+        D ww = mn(D(1), wa[k] < epsD ? D(0) : wa[k]);
+        const D wi = mn(D(1), wia[k] < epsD ? D(0) : wia[k]);
+        if (wi >= D(1) - epsD) ww = D(0);
+        const D t1 = hso[k] * poD[k] * rhowD * ((D(1) - wi) * ww + wi);
+        dsum = (k == 0) ? (t1 - t0) : dsum + (t1 - t0);
+        wa[k] = ww;
+        wia[k] = wi;
+      }
+      gadj = gadj + dsum / dtD;
+    }
+    ST(IBIS_P_HYDROLOGY_GADJUST, gadj);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      ST(IBIS_P_SOIL_TSOI + k, tn[k]);
+      ST(IBIS_P_SOIL_WSOI + k, wa[k]);
+      ST(IBIS_P_SOIL_WISOI + k, wia[k]);
+      ST(IBIS_P_SOIL_POROSFLO + k, porosflo_f[k]);
+    }
+    ST(IBIS_P_SOIL_WPUD, wpf);
+    ST(IBIS_P_SOIL_WIPUD, wipf);
+
+    // ---- fire accumulators (loops.py:1246-1275; fire.py) -----------------------------------
+    {
+      const R swilt1 = LD(IBIS_P_SOIL_SWILT);
+      const R wsoi1 = R(wa[0]), wisoi1 = R(wia[0]);
+      const R water = wisoi1 + (R(1) - wisoi1) * wsoi1;
+      const R frac = (water - swilt1) / (R(1) - swilt1);
+      const R fd = mn(R(1), mx(R(0), R(-2) * (frac - R(0.5))));
+      const R n = mx(R(K.nytimes_new), R(1));
+      const R ff = LD(IBIS_P_VEGETATION_FIREFAC);
+      ST(IBIS_P_VEGETATION_FIREFAC, ((n - R(1)) * ff + fd) / n);
+      const R hit = fd > R(P.d_crit) ? R(1) : R(0);
+      ST(IBIS_P_VEGETATION_NFIRESTEP, LD(IBIS_P_VEGETATION_NFIRESTEP) + hit);
+      ST(IBIS_P_VEGETATION_SDRY, LD(IBIS_P_VEGETATION_SDRY) + fd * hit);
+      const R eact = qa * psurf / (R(0.622) + R(0.378) * qa);
+      const R vpd = mx(esat(ta) - eact, R(0)) / R(1000);
+      const R fv = clip(vpd / R(P.vpd_ref), R(0), R(1));
+      const R hv = fv > R(P.d_crit_vpd) ? R(1) : R(0);
+      ST(IBIS_P_VEGETATION_NFIRESTEP_VPD, LD(IBIS_P_VEGETATION_NFIRESTEP_VPD) + hv);
+      ST(IBIS_P_VEGETATION_SDRY_VPD, LD(IBIS_P_VEGETATION_SDRY_VPD) + fv * hv);
+    }
+  }
+
+  // ---- 12. surface temperature bound (loops.py:1143-1151) and canopy state -------------------
+  const R lo = ta - R(P.t_max_dev), hi = ta + R(P.t_max_dev);
+  ST(IBIS_P_CANOPY_TU, clip(tu, lo, hi));
+  ST(IBIS_P_CANOPY_TS, clip(ts, lo, hi));
+  ST(IBIS_P_CANOPY_TL, clip(tl, lo, hi));
+  ST(IBIS_P_SOIL_TG, clip(tg, lo, hi));
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ST(IBIS_P_CANOPY_WLIQU + k, w[k]);
+}
+
+#undef LD
+#undef ST
+
+}  // namespace ibis

@@ -1,0 +1,109 @@
+"""The hourly step with the reference's signature: ``timestep_body(state, consts, params, dtime)``.
+
+Drop-in for ``python-port/ibis/timestepping/loops.py:234-1284``: forcing is pre-set in
+``state.atm`` (``loops.py:242-243``), the step returns a new state and leaves its input
+untouched (``loops.py:1278-1284``).  ``timestep_`` is the in-place form for drivers that
+own their buffers.  Both run the HIP kernel through the C ABI (``ibis_step`` of
+``include/ibis_step.h``) on the current torch stream of the state's device; there is no
+host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import abi
+from . import layout as L
+from . import synth
+from .params import PhysicalConstants, SimParams
+from .state import IBISState
+
+PRECISIONS = {"mixed": abi.PREC_MIXED, "f32": abi.PREC_F32, "f64": abi.PREC_F64}
+
+_table_cache: dict = {}
+_param_cache: dict = {}
+
+
+def type_table(params: SimParams, device) -> torch.Tensor:
+    """Device copy of the 52-row plant-type Vmax table (cached per device and values)."""
+    vm = np.asarray(params.stomata_params.vmax_pft, dtype=np.float64)
+    if vm.shape != (L.NPFT,):
+        raise ValueError("stomata_params.vmax_pft must be the (12,) base vector; per-cell values "
+                         "come from the state's slot types")
+    ov = tuple(sorted((int(k), float(v)) for k, v in params.vmax_type_overrides.items()))
+    key = (str(device), tuple(vm.tolist()), ov)
+    t = _table_cache.get(key)
+    if t is None:
+        host = synth.type_table(vm, dict(ov))
+        t = torch.from_numpy(host).to(device)
+        _table_cache[key] = t
+    return t
+
+
+def ibis_params(state: IBISState, consts: PhysicalConstants, params: SimParams,
+                flux_max: float = 2000.0, t_max_dev: float = 40.0) -> abi.IbisParams:
+    return abi.params_from(consts, params, state.snow_const, state.hsoi, state.froot,
+                           state.co2conc, flux_max=flux_max, t_max_dev=t_max_dev)
+
+
+def _precision(state: IBISState, precision):
+    if precision is None:
+        precision = "f64" if state.dtype == torch.float64 else "mixed"
+    code = PRECISIONS[precision]
+    want = torch.float64 if code == abi.PREC_F64 else torch.float32
+    if state.dtype != want:
+        raise ValueError(f"precision {precision!r} needs a {want} arena, state holds {state.dtype}")
+    return code
+
+
+def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: float,
+           precision=None, copy_in=False, stream=None, cparams=None):
+    """Enqueue one hour on ``stream`` (default: torch's current stream of the device)."""
+    lib = _lib.load()
+    if state_in.device.type != "cuda":
+        raise RuntimeError("the hourly step runs on the GPU; move the state with "
+                           "IBISState.from_numpy(..., device='cuda')")
+    code = _precision(state_in, precision)
+    if arena_out.shape != state_in.arena.shape or arena_out.dtype != state_in.dtype:
+        raise ValueError("output arena must match the input arena's shape and dtype")
+    if not state_in.arena.is_contiguous() or not arena_out.is_contiguous():
+        raise ValueError("arenas must be contiguous")
+    if state_in.bedrock and params.bedrock is None:
+        raise ValueError("state carries bedrock temperatures but params.bedrock is None "
+                         "(python-port/ibis/physics/bedrock.py:120-124)")
+    flags = code | (abi.FLAG_BEDROCK if state_in.bedrock else 0) | (abi.FLAG_COPY_IN if copy_in else 0)
+    table = type_table(params, state_in.device)
+    p = cparams if cparams is not None else ibis_params(state_in, consts, params)
+    ain = abi.IbisArena(state_in.arena.data_ptr(), state_in.pitch)
+    aout = abi.IbisArena(arena_out.data_ptr(), state_in.pitch)
+    if stream is None:
+        stream = torch.cuda.current_stream(state_in.device)
+    slot = state_in.slot_type
+    rc = lib.ibis_step(C.byref(ain), C.byref(aout),
+                       C.c_void_p(slot.data_ptr() if slot is not None else 0),
+                       C.c_int64(state_in.pitch), C.c_void_p(table.data_ptr()), C.byref(p),
+                       C.c_double(dtime), C.c_double(state_in.nytimes),
+                       C.c_int64(state_in.n), C.c_int(flags), C.c_void_p(stream.cuda_stream))
+    _lib.check(rc, "ibis_step")
+
+
+def timestep_body(state: IBISState, consts: PhysicalConstants, params: SimParams, dtime: float,
+                  precision=None) -> IBISState:
+    """One hour, functional: returns a new state, ``state`` is not modified."""
+    out = torch.empty_like(state.arena)
+    launch(state, out, consts, params, dtime, precision=precision, copy_in=True)
+    new = state.like(out)
+    new.nytimes = state.nytimes + 1.0
+    return new
+
+
+def timestep_(state: IBISState, consts: PhysicalConstants, params: SimParams, dtime: float,
+              precision=None, stream=None, cparams=None) -> IBISState:
+    """One hour in place (the arena is read and written by the same launch)."""
+    launch(state, state.arena, consts, params, dtime, precision=precision, stream=stream,
+           cparams=cparams)
+    state.nytimes = state.nytimes + 1.0
+    return state

@@ -1,0 +1,316 @@
+"""Seeded synthetic land cells and hourly forcing for the step's workloads.
+
+There is no network and no forcing dataset, so every workload is synthetic with the
+distributions of the benchmark configurations:
+
+* 12 plant-type slots per cell, type ids uniform over the 40 climate-zone types;
+  LAI uniform 0-6 on both canopies;
+* the reference default soil layering (``python-port/ibis/io/soil_init.py:17``, 4 m) plus
+  three bedrock layers of (2, 2, 4) m (``python-port/ibis/physics/bedrock.py:35-66``);
+* soil texture: one USDA class per cell, hydraulics from the Rawls et al. (1992) class
+  table (``python-port/ibis/io/soil_init.py:24-31``, pore-space fractions as in
+  ``soil_init.py:214-219``), initial volumetric water U(0.1, 0.45), soil temperature
+  U(260, 300) K, frozen layers carrying ice;
+* hourly forcing: solar geometry from latitude U(-56, 84) deg (``physics/weather.py:519-540``,
+  Hartmann declination), shortwave 1000 W m-2 x cos(zenith) x cloud factor U(0.3, 1.0)
+  split into beam and diffuse by the Spitters (1986) relation (``weather.py:458-480``) and
+  into VIS/NIR 0.46/0.54; air temperature mean U(250, 310) K with an 8 K diurnal amplitude
+  peaking at 15 h; relative humidity U(0.2, 1.0); wind lognormal with mean 3 m/s;
+  precipitation Bernoulli(0.1) x exponential with mean 2 mm/h (snow below 2.5 C as
+  ``weather.py:589-593``); CO2 380 ppm; downward longwave from Idso (1981) as ``weather.py:580-585``.
+
+Random numbers come from a counter-based hash of (seed, stream, cell, hour) -- the
+SplitMix64 finaliser -- so any cell-hour can be drawn alone, in any order, on the host or
+on the device, and a run's result does not depend on how cells are batched.
+"""
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import layout as L
+
+NTYPE = 40
+TMELT = 273.16
+STEF = 5.67e-8
+
+# USDA texture classes: Rawls et al. (1992) table as held by the reference
+# (soil_init.py:24-31) and a representative (sand %, clay %) per class.
+_CLASS_POROS = np.array([0.437, 0.437, 0.453, 0.463, 0.501, 0.398, 0.464, 0.471, 0.430, 0.479, 0.475])
+_CLASS_SFIELD = np.array([0.091, 0.125, 0.207, 0.270, 0.330, 0.255, 0.318, 0.366, 0.339, 0.387, 0.396])
+_CLASS_SWILT = np.array([0.033, 0.055, 0.095, 0.117, 0.133, 0.148, 0.197, 0.208, 0.239, 0.250, 0.272])
+_CLASS_BEX = np.array([1.7, 2.1, 3.1, 4.5, 4.7, 4.0, 5.2, 6.6, 6.0, 7.9, 7.6])
+_CLASS_SUCTION = np.array([0.07, 0.09, 0.15, 0.11, 0.21, 0.28, 0.26, 0.33, 0.29, 0.34, 0.37])
+_CLASS_HYDRAUL = np.array([5.833e-5, 1.6972e-5, 7.1944e-6, 3.6667e-6, 1.8889e-6, 1.1944e-6,
+                           6.3889e-7, 4.1667e-7, 3.3333e-7, 2.5e-7, 1.6667e-7])
+_CLASS_SAND = np.array([92.0, 82.0, 65.0, 42.0, 20.0, 60.0, 32.0, 10.0, 52.0, 7.0, 20.0])
+_CLASS_CLAY = np.array([3.0, 6.0, 10.0, 18.0, 14.0, 27.0, 34.0, 34.0, 42.0, 47.0, 60.0])
+
+HSOI = np.array([0.10, 0.15, 0.25, 0.50, 1.00, 2.00, 0.0])   # 6 layers + the unused 7th entry
+
+# --------------------------------------------------------------------------------------
+# counter-based random numbers
+# --------------------------------------------------------------------------------------
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+_K_SEED = np.uint64(0x9E3779B97F4A7C15)
+_K_STREAM = np.uint64(0xD1B54A32D192ED03)
+_K_CELL = np.uint64(0xAEF17502108EF2D9)
+_K_HOUR = np.uint64(0xB5AD4ECEDA1CE2A9)
+
+
+def _mix64(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, stream, cell, hour=0):
+    """U(0, 1) (open interval) for each (seed, stream, cell, hour); float64."""
+    cell = np.asarray(cell, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) * _K_SEED + np.uint64(stream) * _K_STREAM + cell * _K_CELL
+             + np.uint64(hour) * _K_HOUR)
+    r = _mix64(_mix64(x))
+    return (r >> np.uint64(11)).astype(np.float64) * (2.0 ** -53) + 2.0 ** -54
+
+
+# Streams: one per drawn quantity.
+S_TYPE0 = 100            # .. 111: slot type ids
+S_LAI_L, S_LAI_U, S_SAI_L, S_SAI_U = 1, 2, 3, 4
+S_FU, S_FL, S_ZTOP_L, S_ZTOP_U, S_ZBOT_L, S_ZBOT_U = 5, 6, 7, 8, 9, 10
+S_CLASS, S_THETA, S_TSOI, S_LAT, S_PS = 11, 12, 13, 14, 15
+S_FRAC0 = 200            # .. 211
+S_EXIST0 = 300           # .. 311
+S_FROZEN, S_SAT, S_TFROZ, S_THSAT = 16, 17, 18, 19
+S_TMEAN, S_RH, S_CLOUD = 20, 21, 22          # per cell-day (hour = day index)
+S_WIND1, S_WIND2, S_PWET, S_PAMT = 23, 24, 25, 26   # per cell-hour
+
+
+# --------------------------------------------------------------------------------------
+# plant-type table
+# --------------------------------------------------------------------------------------
+#: parent IBIS PFT (0-based) of each of the 40 zone types, in the order of
+#: python-port/ibis/io/pft_types.py:51-92 (T01..T21 without T20, S01..S07, G01..G08,
+#: W01..W03, C01, C02).
+TYPE_PARENT = np.array([0, 1, 1, 2, 1, 2, 3, 4, 2, 3, 4, 4, 3, 2, 4, 3, 5, 6, 7, 3,
+                        8, 9, 8, 9, 8, 9, 8,
+                        10, 10, 10, 11, 11, 11, 11, 11,
+                        10, 11, 11, 11, 10], dtype=np.int64)
+assert TYPE_PARENT.shape == (NTYPE,)
+
+
+def type_table(vmax_pft, overrides=None):
+    """(NTYPE + 12,) per-row Vmax: rows 0..39 are the zone types (their parent's value unless
+    overridden, as ``pft_types.gather``), rows 40..51 the generic value of each position."""
+    base = np.asarray(vmax_pft, dtype=np.float64)
+    rows = base[TYPE_PARENT].copy()
+    for t, v in (overrides or {}).items():
+        rows[int(t)] = float(v)
+    return np.concatenate([rows, base])
+
+
+def slot_rows(slot_type):
+    """Row of the type table each slot reads: the type id, or 40 + position for a negative
+    (OFF / GENERIC) id."""
+    st = np.asarray(slot_type, dtype=np.int64)
+    pos = np.broadcast_to(np.arange(L.NPFT), st.shape)
+    return np.where(st >= 0, st, NTYPE + pos)
+
+
+def gather_vmax(slot_type, vmax_pft):
+    """Per-cell (N, 12) Vmax from slot type ids."""
+    return type_table(vmax_pft)[slot_rows(slot_type)]
+
+
+# --------------------------------------------------------------------------------------
+# cells
+# --------------------------------------------------------------------------------------
+
+def make_cells(n, seed=0, frozen_frac=0.0, saturated_frac=0.0, cell0=0):
+    """Initial state fields for cells cell0 .. cell0+n-1 (float64, reference shapes).
+
+    Returns (fields, slot_type int8 (n, 12), lat_deg (n,)).
+    """
+    c = np.arange(cell0, cell0 + n, dtype=np.uint64)
+    u = lambda s: uniform(seed, s, c)
+    f = {}
+    slot = np.stack([np.floor(u(S_TYPE0 + j) * NTYPE) for j in range(L.NPFT)], 1).astype(np.int8)
+
+    lai = np.stack([6.0 * u(S_LAI_L), 6.0 * u(S_LAI_U)], 1)
+    sai = np.stack([0.05 + 0.45 * u(S_SAI_L), 0.1 + 0.9 * u(S_SAI_U)], 1)
+    f["vegetation.lai"], f["vegetation.sai"] = lai, sai
+    f["vegetation.fu"] = 0.2 + 0.75 * u(S_FU)
+    f["vegetation.fl"] = 0.2 + 0.75 * u(S_FL)
+    # the upper canopy's base stays above the lower canopy's top (canini's log(z2 - displ),
+    # canopy_exchange.py:257, needs z2 > 0.7 z3)
+    f["vegetation.ztop"] = np.stack([0.5 + 1.0 * u(S_ZTOP_L), 10.0 + 20.0 * u(S_ZTOP_U)], 1)
+    f["vegetation.zbot"] = np.stack([0.05 + 0.25 * u(S_ZBOT_L), 3.0 + 5.0 * u(S_ZBOT_U)], 1)
+    w = np.stack([u(S_FRAC0 + j) for j in range(L.NPFT)], 1)
+    frac = np.zeros_like(w)
+    frac[:, :8] = w[:, :8] / w[:, :8].sum(1, keepdims=True)
+    frac[:, 8:] = w[:, 8:] / w[:, 8:].sum(1, keepdims=True)
+    f["vegetation.frac"] = frac
+    f["vegetation.exist"] = np.stack([(u(S_EXIST0 + j) < 0.6).astype(np.float64)
+                                      for j in range(L.NPFT)], 1)
+    for k in ("firefac", "nfirestep", "sdry", "nfirestep_vpd", "sdry_vpd"):
+        f["vegetation." + k] = np.zeros(n)
+
+    cls = np.minimum((u(S_CLASS) * 11).astype(np.int64), 10)
+    rep = lambda a: np.repeat(a[cls][:, None], L.NSOILAY, axis=1)
+    poros = rep(_CLASS_POROS)
+    f["soil.poros"] = poros
+    f["soil.sfield"] = rep(_CLASS_SFIELD) / poros
+    f["soil.swilt"] = rep(_CLASS_SWILT) / poros
+    f["soil.bex"] = rep(_CLASS_BEX)
+    f["soil.suction"] = rep(_CLASS_SUCTION)
+    f["soil.hydraul"] = rep(_CLASS_HYDRAUL)
+    sand, clay = rep(_CLASS_SAND), rep(_CLASS_CLAY)
+    f["soil.sand"], f["soil.clay"] = sand, clay
+    fs, fc = 0.01 * np.round(sand[:, 0]), 0.01 * np.round(clay[:, 0])
+    albsav = fs * 0.120 + (1.0 - fs - fc) * 0.085 + fc * 0.050
+    f["soil.albsav"], f["soil.albsan"] = albsav, 2.0 * albsav
+    f["soil.z0soi"] = np.full(n, 0.005)
+    f["soil.csoi"] = np.full((n, L.NSOILAY), 880.5)
+    f["soil.rhosoi"] = np.full((n, L.NSOILAY), 2636.5)
+
+    theta = 0.1 + 0.35 * u(S_THETA)
+    tsoi0 = 260.0 + 40.0 * u(S_TSOI)
+    frozen = u(S_FROZEN) < frozen_frac
+    tsoi0 = np.where(frozen, 240.0 + 30.0 * u(S_TFROZ), tsoi0)
+    sat = (~frozen) & (u(S_SAT) < saturated_frac / max(1.0 - frozen_frac, 1e-12))
+    theta = np.where(sat, 0.45 + 0.05 * u(S_THSAT), theta)
+    wtot = np.minimum(theta[:, None] / poros, 1.0)          # total water, fraction of pores
+    depth = np.cumsum(HSOI[:L.NSOILAY]) - 0.5 * HSOI[:L.NSOILAY]
+    # temperature relaxes to the mean with depth; ice where the layer is below freezing
+    tsoi = tsoi0[:, None] + (285.0 - tsoi0[:, None]) * (1.0 - np.exp(-depth / 2.0))[None, :]
+    icefrac = np.clip((TMELT - tsoi) / 5.0, 0.0, 0.95)
+    wisoi = wtot * icefrac
+    wsoi = np.where(wisoi < 1.0, (wtot - wisoi) / np.maximum(1.0 - wisoi, 1e-12), 0.0)
+    f["soil.tsoi"], f["soil.wsoi"], f["soil.wisoi"] = tsoi, np.clip(wsoi, 0.0, 1.0), wisoi
+    f["soil.tg"] = tsoi[:, 0].copy()
+    f["soil.wpud"] = np.zeros(n)
+    f["soil.wipud"] = np.zeros(n)
+    f["soil.tsoi_deep"] = np.repeat(tsoi[:, -1:], L.NBEDROCK, axis=1)
+
+    cold = tsoi0 < TMELT
+    f["snow.fi"] = np.where(cold, 0.5, 0.0)
+    f["snow.hsno"] = np.where(cold[:, None], np.array([0.05, 0.05, 0.05])[None, :], 0.0)
+    f["snow.tsno"] = np.where(cold[:, None], np.minimum(tsoi0, TMELT)[:, None], TMELT) \
+        * np.ones((1, L.NSNOLAY))
+    f["snow.ti"] = np.where(cold, np.minimum(tsoi0, TMELT), TMELT)
+    f["snow.tlsub"] = np.where(cold, np.minimum(tsoi0, TMELT), TMELT)
+
+    # canopy temperatures and humidities start at the first day's mean air state
+    tair0 = 250.0 + 60.0 * uniform(seed, S_TMEAN, c, 0)
+    psurf0 = 101325.0 * (0.85 + 0.15 * u(S_PS))
+    q0 = (0.2 + 0.8 * uniform(seed, S_RH, c, 0)) * _qsat(_esat(tair0), psurf0)
+    for k in ("tu", "ts", "tl", "t12", "t34"):
+        f["canopy." + k] = tair0.copy()
+    f["canopy.q12"] = q0.copy()
+    f["canopy.q34"] = q0.copy()
+    for k in ("wliqu", "wliqs", "wliql", "wsnou", "wsnos", "wsnol"):
+        f["canopy." + k] = np.zeros(n)
+    for k, w_ in (("upper", 2), ("lower", 3)):
+        f[f"canopy.ci_{k}"] = np.full((n, w_), 350.0e-6)
+        f[f"canopy.gs_{k}"] = np.full((n, w_), 0.01)
+        f[f"canopy.cs_{k}"] = np.full((n, w_), 350.0e-6)
+    f["canopy.a10scalparamu"] = np.full(n, 0.5)
+    f["canopy.a10daylightu"] = np.full(n, 100.0)
+    f["canopy.a10scalparaml"] = np.full(n, 0.5)
+    f["canopy.a10daylightl"] = np.full(n, 100.0)
+
+    lat = -56.0 + 140.0 * u(S_LAT)
+    f["atm.psurf"] = psurf0
+    return f, slot, lat
+
+
+def _declination(jday):
+    orbit = 2.0 * 3.1415927 * jday / 365.2425
+    return (0.006918 - 0.399912 * np.cos(orbit) + 0.070257 * np.sin(orbit)
+            - 0.006758 * np.cos(2.0 * orbit) + 0.000907 * np.sin(2.0 * orbit)
+            - 0.002697 * np.cos(3.0 * orbit) + 0.001480 * np.sin(3.0 * orbit))
+
+
+def spitters_diffuse(kt, coszen):
+    """Diffuse fraction of an hourly total from the clearness index (weather.py:458-480)."""
+    r = 0.847 - 1.61 * coszen + 1.04 * coszen * coszen
+    kk = (1.47 - r) / 1.66
+    fd = np.where(kt <= 0.22, 1.0, np.where(kt <= 0.35, 1.0 - 6.4 * (kt - 0.22) ** 2,
+                                             np.where(kt <= kk, 1.47 - 1.66 * kt, r)))
+    return np.where(coszen <= 0.0, 1.0, np.clip(fd, 0.0, 1.0))
+
+
+def forcing(n, seed, hour, lat, psurf, jday0=172, cell0=0):
+    """Atmospheric forcing of hour ``hour`` (0-based from the run start) for each cell."""
+    c = np.arange(cell0, cell0 + n, dtype=np.uint64)
+    day = hour // 24
+    h = hour % 24
+    jday = jday0 + day
+    decl = _declination(jday)
+    angle = 2.0 * 3.1415927 * (h - 12.0) / 24.0
+    xlat = lat * 3.1415927 / 180.0
+    coszen = np.maximum(0.0, np.sin(xlat) * np.sin(decl) + np.cos(xlat) * np.cos(decl) * np.cos(angle))
+    cf = 0.3 + 0.7 * uniform(seed, S_CLOUD, c, day)
+    sw = 1000.0 * coszen * cf
+    orbit = 2.0 * 3.1415927 * jday / 365.2425
+    s0 = 1370.0 * (1.000110 + 0.034221 * np.cos(orbit) + 0.001280 * np.sin(orbit)
+                   + 0.000719 * np.cos(2.0 * orbit) + 0.000077 * np.sin(2.0 * orbit))
+    kt = np.clip(sw / np.maximum(s0 * coszen, 1.0), 0.0, 1.0)
+    fd = spitters_diffuse(kt, coszen)
+    band = np.array([0.46, 0.54])
+    solad = sw[:, None] * band[None, :] * (1.0 - fd)[:, None]
+    solai = sw[:, None] * band[None, :] * fd[:, None]
+
+    tmean = 250.0 + 60.0 * uniform(seed, S_TMEAN, c, day)
+    ta = tmean + 8.0 * np.cos(2.0 * np.pi * (h - 15.0) / 24.0)
+    rh = 0.2 + 0.8 * uniform(seed, S_RH, c, day)
+    qa = rh * _qsat(_esat(ta), psurf)
+    wind_mu = np.log(3.0) - 0.5 * 0.5 * 0.5
+    z1 = uniform(seed, S_WIND1, c, hour)
+    z2 = uniform(seed, S_WIND2, c, hour)
+    gauss = np.sqrt(-2.0 * np.log(z1)) * np.cos(2.0 * np.pi * z2)
+    ua = np.exp(wind_mu + 0.5 * gauss)
+    wet = uniform(seed, S_PWET, c, hour) < 0.1
+    amount = -2.0 * np.log(uniform(seed, S_PAMT, c, hour))      # mm/h, exponential mean 2
+    rate = np.where(wet, amount / 3600.0, 0.0)                  # kg m-2 s-1
+    warm = (ta - 273.15) > 2.5
+    raina = np.where(warm, rate, 0.0)
+    snowa = np.where(warm, 0.0, rate)
+    cloud = 1.0 - cf
+    emb = 0.01 * (psurf * qa / (0.622 + qa))
+    ea = 0.70 + 5.95e-5 * emb * np.exp(1500.0 / ta)
+    t4 = (ta * ta) * (ta * ta)
+    fira = (1.0 - cloud) * ea * STEF * t4 + cloud * 0.950 * STEF * t4
+    return {"atm.ta": ta, "atm.qa": qa, "atm.ua": ua, "atm.psurf": psurf,
+            "atm.coszen": coszen, "atm.solad": solad, "atm.solai": solai, "atm.fira": fira,
+            "atm.raina": raina, "atm.snowa": snowa}
+
+
+def _esat(t):
+    """Lowe (1977) saturation vapour pressure, Pa (``physics/thermo.py:65-97``)."""
+    tl = np.clip(t - TMELT, 0.0, 100.0)
+    ti = np.clip(t - TMELT, -60.0, 0.0)
+    a = (6.1078000, 4.4365185e-1, 1.4289458e-2, 2.6506485e-4, 3.0312404e-6, 2.0340809e-8,
+         6.1368209e-11)
+    b = (6.1091780, 5.0346990e-1, 1.8860134e-2, 4.1762237e-4, 5.8247203e-6, 4.8388032e-8,
+         1.8388269e-10)
+    pa = a[6]
+    pb = b[6]
+    for j in (5, 4, 3, 2, 1):
+        pa = a[j] + tl * pa
+        pb = b[j] + ti * pb
+    return 100.0 * (np.where(t >= TMELT, a[0], b[0]) + tl * pa + ti * pb)
+
+
+def _qsat(e, p):
+    return 0.622 * e / np.maximum(p - 0.378 * e, 0.622 * e)
+
+
+def make_workload(n, seed=0, frozen_frac=0.0, saturated_frac=0.0, cell0=0, hour=0):
+    """Cells plus the forcing of ``hour``: (fields, slot_type, lat)."""
+    f, slot, lat = make_cells(n, seed, frozen_frac, saturated_frac, cell0)
+    f.update(forcing(n, seed, hour, lat, f["atm.psurf"], cell0=cell0))
+    return f, slot, lat

@@ -1,0 +1,115 @@
+/*
+ * C ABI of the MI355X hourly land-surface step (IBIS 3.0 timestep_body).
+ *
+ * Replaces the jitted JAX step
+ *   timestep_body(state, consts, params, dtime) -> state
+ * of python-port/ibis/timestepping/loops.py:234-1284, called once per hour from the
+ * lax.scan bodies python-port/ibis/timestepping/loops.py:1334-1367 and
+ * python-port/ibis/timestepping/loops_gswp3.py:115-154.
+ *
+ * Plain pointers and sizes only.  All pointers except `params` are device pointers; the
+ * call is stream-ordered, allocates nothing and never synchronises, so it can be captured
+ * into a HIP graph.  Return value: 0 on success, a negative IBIS_E_* code on a
+ * configuration error (the reference raises ValueError for the same cases,
+ * bedrock.py:120-124); numerical trouble is not an error (NaN/Inf propagate as in the
+ * reference, bounded by its clamps).
+ */
+#ifndef IBIS_STEP_H
+#define IBIS_STEP_H
+
+#include <stdint.h>
+#include "ibis_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Packed state: IBIS_NPLANE planes of `pitch` elements (float for IBIS_PREC_MIXED and
+ * IBIS_PREC_F32, double for IBIS_PREC_F64), plane order of ibis_layout.h.  A reference
+ * leaf of shape (npoi, k) is planes [IBIS_P_<leaf>, IBIS_P_<leaf> + k) -- the IBISState
+ * pytree of python-port/ibis/state/model_state.py:28-63 restricted to the leaves the step
+ * touches. */
+typedef struct IbisArena {
+  void* planes;
+  int64_t pitch;
+} IbisArena;
+
+/* Scalar parameters, by value.  Field names are the reference's:
+ *   PhysicalConstants  python-port/ibis/constants.py:16-79
+ *   StomataParams      python-port/ibis/physics/stomata.py:33-161 (F77 switches)
+ *   CanopyParams       python-port/ibis/physics/canopy_exchange.py:68-84
+ *   InterceptionParams python-port/ibis/physics/interception.py:31-45
+ *   VegRadParams       python-port/ibis/physics/radiation.py:32-56
+ *   SimParams soil     python-port/ibis/timestepping/loops.py:86-105
+ *   BedrockParams      python-port/ibis/physics/bedrock.py:35-66
+ *   FireParams         python-port/ibis/physics/fire.py:203-270
+ *   SnowState statics  python-port/ibis/state/snow.py:34-40
+ *   hsoi, froot        SoilState.hsoi (nsoilay+1,), VegetationState.froot (nsoilay, 2)
+ *   co2conc            AtmState.co2conc
+ *   flux_max, t_max_dev  IBIS_FLUX_MAX / IBIS_T_MAX_DEV of loops.py:665-666, 1143-1144 */
+typedef struct IbisParams {
+  double stef, vonk, grav, tmelt, hvap, hfus, hsub, ch2o, cice, cair, cvap, rair, rvap,
+      cappa, rhow, epsilon;
+  double tau15, kc15, ko15, cimax, o2conc, alpha3, theta3, beta3, alpha4, theta4, beta4;
+  double gamma[5], coefm[5], coefb[5], gsmin[5]; /* broadleaf, conifer, shrub, C4, C3 */
+  double stressfac;
+  double cleaf, cstem, cgrass, dleaf[2], dstem[2], alaiml, alaimu, chu, chs, chl,
+      soil_beta_power;
+  double wliqumax, wliqsmax, wliqlmax, wsnoumax, wsnosmax, wsnolmax, tdripu, tdrips, tdripl,
+      tblowu, tblows, tblowl, fmax;
+  double rhoveg[2][2], tauveg[2][2], oriev[2], orieh[2]; /* [band][story], story 0 lower */
+  double wpudmax, zwpmax, bperm, rhosoi_val, csoi_mineral_val;
+  int32_t h2o_consistent_weights, pad0;
+  double bed_hsoi[IBIS_NBEDROCK], bed_consoi, bed_rhosoi, bed_csoi;
+  double d_crit, vpd_ref, d_crit_vpd;
+  double z0sno, rhos, consno, hsnotop, hsnomin, fimin, fimax;
+  double flux_max, t_max_dev;
+  double hsoi[IBIS_NSOILAY + 1];
+  double froot[IBIS_NSOILAY][2];
+  double co2conc;
+} IbisParams;
+
+/* Rows of the plant-type table: 40 Koppen-zone types, then the 12 generic positions.
+ * A slot holding type t (0..39) reads row t; a negative slot id (OFF / GENERIC,
+ * python-port/ibis/io/pft_types.py:29) at position j reads row 40 + j
+ * (python-port/ibis/io/pft_types.py:170-189).  Column 0 is Vmax (mol m-2 s-1). */
+#define IBIS_NTYPE 40
+#define IBIS_TYPE_ROWS (IBIS_NTYPE + IBIS_NPFT)
+#define IBIS_TYPE_COLS 1
+
+/* Precision of storage / bulk arithmetic / tridiagonal and 9x9 solves. */
+#define IBIS_PREC_MIXED 0 /* f32 / f32 / f64 */
+#define IBIS_PREC_F32 1   /* f32 / f32 / f32 */
+#define IBIS_PREC_F64 2   /* f64 / f64 / f64 */
+#define IBIS_PREC_MASK 0x3
+/* The state carries soil.tsoi_deep: solve heat through the thermal-only bedrock. */
+#define IBIS_FLAG_BEDROCK 0x10
+/* Also copy the read-only planes from `in` to `out` (functional, out-of-place use). */
+#define IBIS_FLAG_COPY_IN 0x20
+
+#define IBIS_OK 0
+#define IBIS_E_ARG (-1)      /* null pointer, negative size, pitch < ncell */
+#define IBIS_E_FLAGS (-2)    /* unknown precision or flag bits */
+#define IBIS_E_LAUNCH (-3)   /* the HIP launch failed */
+#define IBIS_E_BEDROCK (-4)  /* bedrock layer count / thicknesses inconsistent */
+
+/* One hour for `ncell` cells: reads the planes of `in`, writes every IO/OUT plane of `out`
+ * (out may equal in).  slot_type: IBIS_NPFT int8 planes of `slot_pitch` (NULL: every slot
+ * generic).  type_table: IBIS_TYPE_ROWS x IBIS_TYPE_COLS doubles, device memory.
+ * nytimes: VegetationState.nytimes before the step (the step advances it by one).
+ * Replaces timestep_body (python-port/ibis/timestepping/loops.py:234-1284). */
+int ibis_step(const IbisArena* in, const IbisArena* out, const int8_t* slot_type,
+              int64_t slot_pitch, const double* type_table, const IbisParams* params,
+              double dtime, double nytimes, int64_t ncell, int flags, void* hip_stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ibis_last_error(void);
+
+/* Library version string. */
+const char* ibis_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IBIS_STEP_H */

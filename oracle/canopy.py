@@ -1,0 +1,382 @@
+"""Canopy aerodynamics and the 9x9 energy/vapour solve (FP64), restating
+``python-port/ibis/physics/canopy_exchange.py``.
+
+Pinned by: the bare-cell identity rows giving t12 = t34 = ta*tfac and q12 = q34 = qa
+(``canopy_exchange.py:709-747``); the solved vector satisfying the assembled system
+(residual of ``numpy.linalg.solve``); the sensible and latent flux closure
+fsena = fsenu*xu + fsens*xs + fsenl*xl + fseng*(1-fi) + fseni*fi at the solution (row 3/4
+conservation) and fvapa = fvapuw*xu + fvaput*... (rows 5/6); impexp weights in [0, 1].
+``canini``/``turcof``/``fstrat`` values have only sanity checks in the reference
+(``tests/test_canopy.py:41-185``): parity unpinned beyond those invariants.
+"""
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+import numpy as np
+
+from ._num import ipow
+from .solvers import gauss_nopivot
+from .thermo import esat, qsat, dqsat, hvapf, hsubf
+
+
+def canini(ta, qa, psurf, ua, fi, z0soi, z0sno, fl, fu, lai, sai, ztop, zbot, cp_, c):
+    """Aerodynamic geometry (canopy_exchange.py:192-280)."""
+    siga = 0.999
+    tfac = 1.0 / (siga ** c.cappa)
+    pa = psurf * siga
+    rhoa = pa / (c.rair * ta * (1.0 + (c.rvap / c.rair - 1.0) * qa))
+    cp = c.cair * (1.0 + (c.cvap / c.cair - 1.0) * qa)
+    za = (psurf - pa) / (rhoa * c.grav)
+    za = np.maximum(za, ztop[:, 1] + 1.0)
+
+    x = fl * (1.0 - fi) * 2.0 * (lai[:, 0] + sai[:, 0]) / cp_.alaiml
+    x = np.minimum(x, 3.0)
+    x1 = np.minimum(x, 1.0)
+    rw = np.maximum(ztop[:, 0] - zbot[:, 0], 0.01)
+    cvegl = (0.4 / rw) * np.maximum(1e-5, x)
+    dvegl = (0.1 * rw) / np.maximum(1e-5, np.maximum(x, x * x))
+    bvegl = np.sqrt(2.0 * cvegl / dvegl)
+    bdl = 0.5 * bvegl * dvegl
+    dil = 1.0 / dvegl
+    rw = (1.0 - x1) * (np.maximum(z0soi, z0sno) + 0.01)
+    z3 = x1 * ztop[:, 0] + rw
+    z4 = x1 * zbot[:, 0] + rw
+    z34 = 0.5 * (z3 + z4)
+    exphl = np.exp(0.5 * bvegl * (z3 - z4))
+    expl = exphl * exphl
+    displ = x1 * 0.7 * z3
+
+    x = fu * 2.0 * (lai[:, 1] + sai[:, 1]) / cp_.alaimu
+    x = np.minimum(x, 3.0)
+    x1 = np.minimum(x, 1.0)
+    rw = np.maximum(ztop[:, 1] - zbot[:, 1], 0.01)
+    cvegu = (0.4 / rw) * np.maximum(1e-5, x)
+    dvegu = (0.1 * rw) / np.maximum(1e-5, np.maximum(x, x * x))
+    rw = 1.0 / dvegu
+    bvegu = np.sqrt(2.0 * cvegu * rw)
+    bdu = 0.5 * bvegu * dvegu
+    diu = rw
+    rw = (1.0 - x1) * (z3 + 0.01)
+    z1 = x1 * ztop[:, 1] + rw
+    z2 = x1 * zbot[:, 1] + rw
+    z12 = 0.5 * (z1 + z2)
+    exphu = np.exp(0.5 * bvegu * (z1 - z2))
+    expu = exphu * exphu
+    dispu = x1 * 0.7 * z1 + (1.0 - x1) * displ
+
+    alogg = np.log(z0soi)
+    alogi = np.log(z0sno)
+    alogav = (1.0 - fi) * alogg + fi * alogi
+    alog4 = np.log(np.maximum(z4, np.maximum(1.1 * z0soi, 1.1 * z0sno)))
+    alog3 = np.log(z3 - displ)
+    alog2 = np.log(z2 - displ)
+    alog1 = np.log(z1 - dispu)
+    aloga = np.log(za - dispu)
+    u2 = ua / exphu
+    alogu = np.log(np.maximum(0.01, 0.1 * (z1 - z2)))
+    alogl = np.log(np.maximum(0.01, 0.1 * (z3 - z4)))
+    return SimpleNamespace(rhoa=rhoa, cp=cp, za=za, tfac=tfac, z1=z1, z2=z2, z3=z3, z4=z4,
+                           z12=z12, z34=z34, bdl=bdl, bdu=bdu, dil=dil, diu=diu, exphl=exphl,
+                           expl=expl, exphu=exphu, expu=expu, displ=displ, dispu=dispu,
+                           alogav=alogav, alog1=alog1, alog2=alog2, alog3=alog3, alog4=alog4,
+                           aloga=aloga, alogu=alogu, alogl=alogl, u2=u2)
+
+
+def fstrat(tb, tt, ttfac, qb, qt, zb, zt, albm, albh, alt, u, c, first, stram_p, strah_p):
+    """Louis (1979) stability factors (canopy_exchange.py:287-326)."""
+    zht = tt * ttfac * (1.0 + 0.622 * qt)
+    zhb = tb * (1.0 + 0.622 * qb)
+    rich = c.grav * np.maximum(zt - zb, 0.0) * (zht - zhb) / (0.5 * (zht + zhb) * (u * u))
+    rich = np.clip(rich, -2.0, 1.0)
+    xm = np.maximum(alt - albm, 0.5)
+    xh = np.maximum(alt - albh, 0.5)
+    rw = np.sqrt(np.maximum(-rich, 0.0))
+    ym = ((c.vonk / xm) * (c.vonk / xm)) * np.exp(0.5 * xm) * rw
+    yh = ((c.vonk / xh) * (c.vonk / xh)) * np.exp(0.5 * xh) * rw
+    m_un = 1.0 - 10.0 * rich / (1.0 + 75.0 * ym)
+    h_un = 1.0 - 15.0 * rich / (1.0 + 75.0 * yh)
+    rp = np.maximum(rich, 0.0)
+    zz = np.sqrt(1.0 + 5.0 * rp)
+    m_st = 1.0 / (1.0 + 10.0 * rp / zz)
+    h_st = 1.0 / (1.0 + 15.0 * rp * zz)
+    stram = np.where(rich <= 0.0, m_un, m_st)
+    strah = np.where(rich <= 0.0, h_un, h_st)
+    if not first:
+        stram = 0.5 * stram + 0.5 * stram_p
+        strah = 0.5 * strah + 0.5 * strah_p
+    return rich, stram, strah
+
+
+def turcof(aero, ta, qa, ua, t12, t34, q12, q34, c, cp_, dtime, first, carry):
+    """Wind profile and transfer coefficients (canopy_exchange.py:333-426).
+
+    ``carry`` holds straml/strahl/stramu/strahu of the previous pass.
+    """
+    vonk = c.vonk
+    _, straml, strahl = fstrat(t34, t12, 1.0, q34, q12, aero.z3, aero.z2, aero.alogl,
+                               aero.alogl, aero.alog2, aero.u2, c, first, carry.straml,
+                               carry.strahl)
+    _, stramu, strahu = fstrat(t12, ta, aero.tfac, q12, qa, aero.z1, aero.za, aero.alogu,
+                               aero.alogu, aero.aloga, ua, c, first, carry.stramu, carry.strahu)
+    x = ((aero.alog4 - aero.alogav) / vonk)
+    x = (x * x) * aero.bdl
+    rw = 1.0 / aero.expl
+    yl = ((x + 1) * aero.expl + (x - 1) * rw) / ((x + 1) * aero.expl - (x - 1) * rw)
+    alogl = aero.alog3 - vonk * np.sqrt(yl / aero.bdl)
+    x = ((aero.alog2 - alogl) / vonk)
+    x = (x * x) * aero.bdu / straml
+    rw = 1.0 / aero.expu
+    yu = ((x + 1) * aero.expu + (x - 1) * rw) / ((x + 1) * aero.expu - (x - 1) * rw)
+    alogu = aero.alog1 - vonk * np.sqrt(yu / aero.bdu)
+    cdmax = 300.0 / (2.0 * dtime)
+    t = (ua * vonk / (aero.aloga - alogu))
+    tauu = (t * t) * stramu
+    a = 0.5 * tauu * (yu + 1) / aero.bdu
+    b = 0.5 * tauu * (yu - 1) / aero.bdu
+    taul = aero.bdu * (a / aero.expu - b * aero.expu)
+    cc = 0.5 * taul * (yl + 1) / aero.bdl
+    d = 0.5 * taul * (yl - 1) / aero.bdl
+    us = lambda v: np.maximum(0.01, np.sqrt(np.maximum(0.0, v)))
+    u1 = us(a + b)
+    u12 = us(a / aero.exphu + b * aero.exphu)
+    u2 = us(a / aero.expu + b * aero.expu)
+    u3 = us(cc + d)
+    u34 = us(cc / aero.exphl + d * aero.exphl)
+    u4 = us(cc / aero.expl + d * aero.expl)
+    ca = ua * strahu * (vonk * vonk) / ((aero.aloga - alogu) * (aero.aloga - aero.alog1))
+    ca = np.minimum(cdmax, ca / (1.0 + ca * 1e-20))
+    cai = 1.0 / (aero.rhoa * ca)
+    cbi = aero.diu * (aero.z1 - aero.z12) / (aero.rhoa * 0.5 * (u1 + u12))
+    cci = aero.diu * (aero.z12 - aero.z2) / (aero.rhoa * 0.5 * (u12 + u2))
+    cdi = (aero.alog2 - alogl) * (aero.alog2 - aero.alog3) / (aero.rhoa * u2 * strahl
+                                                                * (vonk * vonk))
+    cei = aero.dil * (aero.z3 - aero.z34) / (aero.rhoa * 0.5 * (u3 + u34))
+    cfi = aero.dil * (aero.z34 - aero.z4) / (aero.rhoa * 0.5 * (u34 + u4))
+    cu = 1.0 / (cai + cbi)
+    cl = 1.0 / (cci + cdi + cei)
+    su = aero.rhoa * cp_.cleaf * np.sqrt(u12 / cp_.dleaf[1])
+    ss = aero.rhoa * cp_.cstem * np.sqrt(u12 / cp_.dstem[1])
+    sl = aero.rhoa * cp_.cgrass * np.sqrt(u34 / cp_.dleaf[0])
+    sg0 = aero.rhoa * (0.004 + 0.012 * u4)
+    si0 = aero.rhoa * (0.003 + 0.010 * u4)
+    sg = 1.0 / (cfi + 1.0 / sg0)
+    si = 1.0 / (cfi + 1.0 / si0)
+    return SimpleNamespace(cu=cu, cl=cl, su=su, ss=ss, sl=sl, sg=sg, si=si, u2=u2, alogu=alogu,
+                           alogl=alogl, straml=straml, strahl=strahl, stramu=stramu,
+                           strahu=strahu)
+
+
+def impexp(tveg, ch, wliq, wsno, c, prev, first):
+    """Implicit weight near the melt point (canopy_exchange.py:433-445)."""
+    if first:
+        return np.ones_like(tveg)
+    eps = c.epsilon
+    h = ch * (tveg - c.tmelt)
+    z = np.maximum(np.abs(h), eps)
+    wn = np.where(h > eps + 1.0e-8, 1.0 - np.minimum(1.0, c.hfus * wsno / z),
+                  np.where(h < -eps - 1.0e-8, 1.0 - np.minimum(1.0, c.hfus * wliq / z), 1.0))
+    return 0.5 * (prev + wn)
+
+
+def impexp2(t, told, c, prev, first):
+    """Implicit weight of the snow skin temperature (canopy_exchange.py:448-463)."""
+    if first:
+        return np.ones_like(t)
+    dt = t - told
+    warm = dt > c.epsilon + 1.0e-8
+    wn = np.where(warm, (c.tmelt - told) / np.where(warm, dt, 1.0), prev)
+    return np.clip(wn, 0.0, 1.0)
+
+
+def turvap(s, c, dtime, beta_power=1.0):
+    """Assemble and solve the 9x9 implicit canopy system (canopy_exchange.py:470-811).
+
+    ``s`` carries every input by the reference's argument names.
+    """
+    eps = c.epsilon
+    wu, ws, wl, wg, wi = s.wu, s.ws, s.wl, s.wg, s.wi
+    tu = wu * s.tu + (1.0 - wu) * s.tuold
+    ts = ws * s.ts + (1.0 - ws) * s.tsold
+    tl = wl * s.tl + (1.0 - wl) * s.tlold
+    tg = wg * s.tg + (1.0 - wg) * s.tgold
+    ti = wi * s.ti + (1.0 - wi) * s.tiold
+    ps = s.psurf
+
+    def sat(t):
+        e = esat(t)
+        q = qsat(e, ps)
+        return q, np.minimum(dqsat(t, q, e), q * 0.1)
+    qu, dqu = sat(tu)
+    qs, dqs = sat(ts)
+    ql, dql = sat(tl)
+    qg, dqg = sat(tg)
+    qi, dqi = sat(ti)
+
+    zwpud = np.clip(0.5 * (s.wpud + s.wipud) / max(s.wpudmax, eps), 0.0, 0.5)
+    zwsoi = (1.0 - s.wisoi1) * s.wsoi1 + s.wisoi1
+    zwtot = zwpud + (1.0 - zwpud) * zwsoi
+    psig = -c.grav * s.suction1 * (zwtot ** (-s.bex1))
+    with np.errstate(over="ignore"):
+        hfac = np.exp(psig / (c.rvap * tg))
+    hfac2 = s.q34 / np.maximum(qg, eps)
+    betaw = np.clip((zwtot - s.swilt1) / np.maximum(1.0 - s.swilt1, eps), 0.0, 1.0)
+    if beta_power != 1.0:
+        betaw = betaw ** beta_power
+    e34 = esat(s.t34)
+    qs1 = qsat(e34, ps)
+    dqs1 = dqsat(s.t34, qs1)
+    xnumer = c.hvap * dqs1
+    xdenom = s.cp + (4.0 * 0.95 * c.stef * ipow(s.t34, 3)) / s.sg
+    betafac = xnumer / xdenom
+    betas = betaw / (1.0 + betafac * (1.0 - betaw))
+    with np.errstate(invalid="ignore"):
+        qgfac0 = np.where(betas > 0.0, betas * hfac + (1.0 - betas) * hfac2, hfac2)
+    qeps = 1.0e-10
+    qgfac = np.where(s.q34 > qg + qeps, 1.0, qgfac0)
+    fwetux = np.where(s.q12 > qu + qeps, 1.0, s.fwetu)
+    fwetsx = np.where(s.q12 > qs + qeps, 1.0, s.fwets)
+    fwetlx = np.where(s.q34 > ql + qeps, 1.0, s.fwetl)
+
+    lai, sai = s.lai, s.sai
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fradu = np.where(lai[:, 1] > eps, (s.solu + s.firu) / (2.0 * lai[:, 1]), 0.0)
+        frads = np.where(sai[:, 1] > eps, (s.sols + s.firs) / (2.0 * sai[:, 1]), 0.0)
+        fradl = np.where((lai[:, 0] + sai[:, 0]) > eps,
+                         (s.soll + s.firl) / (2.0 * (lai[:, 0] + sai[:, 0])), 0.0)
+
+    hv = lambda t: hvapf(t, s.ta, c)
+    hs = lambda t: hsubf(t, s.ta, c)
+    fr = s.frac
+    suw = np.minimum(fwetux * s.su,
+                     0.8 * (s.wliqu + s.wsnou) / np.maximum(dtime * (qu - s.q12), eps))
+    sut = np.maximum(0.0, (1.0 - fwetux) * 0.5 * (
+        s.totcondub * fr[:, 0] + s.totcondub * fr[:, 1] + s.totcondub * fr[:, 2]
+        + s.totconduc * fr[:, 3] + s.totcondub * fr[:, 4] + s.totconduc * fr[:, 5]
+        + s.totcondub * fr[:, 6] + s.totcondub * fr[:, 7]))
+    suh = (suw * (s.rliqu * hv(tu) + (1.0 - s.rliqu) * hs(tu)) + sut * hv(tu))
+    ssw = np.minimum(fwetsx * s.ss,
+                     0.8 * (s.wliqs + s.wsnos) / np.maximum(dtime * (qs - s.q12), eps))
+    ssh = ssw * (s.rliqs * hv(ts) + (1.0 - s.rliqs) * hs(ts))
+    slw = np.minimum(fwetlx * s.sl,
+                     0.8 * (s.wliql + s.wsnol) / np.maximum(dtime * (ql - s.q34), eps))
+    slt0 = np.maximum(0.0, (1.0 - fwetlx) * 0.5 * (
+        s.totcondls * fr[:, 8] + s.totcondls * fr[:, 9] + s.totcondl4 * fr[:, 10]
+        + s.totcondl3 * fr[:, 11]))
+    slt = slt0 * lai[:, 0] / np.maximum(lai[:, 0] + sai[:, 0], eps)
+    slh = (slw * (s.rliql * hv(tl) + (1.0 - s.rliql) * hs(tl)) + slt * hv(tl))
+
+    n = tu.shape[0]
+    A = np.zeros((n, 9, 9))
+    r = np.zeros((n, 9))
+    rw = 1.0 / dtime
+    cp, fi, xu, xs, xl = s.cp, s.fi, s.xu, s.xs, s.xl
+    su, ss, sl, sg, si, cu, cl = s.su, s.ss, s.sl, s.sg, s.si, s.cu, s.cl
+
+    w2 = su * cp
+    A[:, 0, 0] = s.chux * rw + wu * w2 + wu * suh * dqu
+    A[:, 0, 3] = -w2
+    A[:, 0, 5] = -suh
+    r[:, 0] = (s.tuold * s.chux * rw - (1.0 - wu) * w2 * tu - suh * (qu - wu * dqu * tu)
+               + fradu - s.pfluxu)
+    w2 = ss * cp
+    A[:, 1, 1] = s.chsx * rw + ws * w2 + ws * ssh * dqs
+    A[:, 1, 3] = -w2
+    A[:, 1, 5] = -ssh
+    r[:, 1] = (s.tsold * s.chsx * rw - (1.0 - ws) * w2 * ts - ssh * (qs - ws * dqs * ts)
+               + frads - s.pfluxs)
+    w2 = sl * cp
+    A[:, 2, 2] = s.chlx * rw + wl * w2 + wl * slh * dql
+    A[:, 2, 4] = -w2
+    A[:, 2, 6] = -slh
+    r[:, 2] = (s.tlold * s.chlx * rw - (1.0 - wl) * w2 * tl - slh * (ql - wl * dql * tl)
+               + fradl - s.pfluxl)
+    ru, rs_ = xu * su, xs * ss
+    A[:, 3, 0] = -wu * ru
+    A[:, 3, 1] = -ws * rs_
+    A[:, 3, 3] = cu + cl + ru + rs_
+    A[:, 3, 4] = -cl
+    r[:, 3] = cu * s.ta * s.tfac + (1.0 - wu) * ru * tu + (1.0 - ws) * rs_ * ts
+    rl, ri = xl * sl, fi * si
+    A[:, 4, 2] = -wl * rl
+    A[:, 4, 3] = -cl
+    A[:, 4, 4] = cl + rl + (1.0 - fi) * sg + ri
+    A[:, 4, 7] = -wg * (1.0 - fi) * sg
+    A[:, 4, 8] = -wi * ri
+    r[:, 4] = (1.0 - wl) * rl * tl + (1.0 - wg) * (1.0 - fi) * sg * tg + (1.0 - wi) * ri * ti
+    ruq, rsq = xu * (suw + sut), xs * ssw
+    A[:, 5, 0] = -wu * ruq * dqu
+    A[:, 5, 1] = -ws * rsq * dqs
+    A[:, 5, 5] = cu + cl + ruq + rsq
+    A[:, 5, 6] = -cl
+    r[:, 5] = cu * s.qa + ruq * (qu - wu * dqu * tu) + rsq * (qs - ws * dqs * ts)
+    rlq, rgq = xl * (slw + slt), (1.0 - fi) * sg
+    A[:, 6, 2] = -wl * rlq * dql
+    A[:, 6, 5] = -cl
+    A[:, 6, 6] = cl + rlq + rgq + fi * si
+    A[:, 6, 7] = -wg * rgq * qgfac * dqg
+    A[:, 6, 8] = -wi * fi * si * dqi
+    r[:, 6] = (rlq * (ql - wl * dql * tl) + rgq * qgfac * (qg - wg * dqg * tg)
+               + fi * si * (qi - wi * dqi * ti))
+    a1, a2 = sg * cp, sg * s.hvasug
+    A[:, 7, 4] = -a1
+    A[:, 7, 6] = -a2
+    A[:, 7, 7] = a1 + a2 * qgfac * dqg + s.cog + s.zirg
+    r[:, 7] = -a2 * qgfac * (qg - dqg * tg) + s.cog * s.tsoi1 + s.solg + s.firg + s.zirg * s.tgold
+    a1, a2 = si * cp, si * s.hvasui
+    A[:, 8, 4] = -a1
+    A[:, 8, 6] = -a2
+    A[:, 8, 8] = a1 + a2 * dqi + s.coi + s.ziri
+    r[:, 8] = -a2 * (qi - dqi * ti) + s.coi * s.tsno1 + s.soli + s.firi + s.ziri * s.tiold
+
+    bare = (xu + xs + xl) < eps
+    if np.any(bare):
+        A[bare, :7, :] = 0.0
+        for j in range(7):
+            A[bare, j, j] = 1.0
+        tt = s.ta * s.tfac
+        for j in range(5):
+            r[bare, j] = tt[bare]
+        r[bare, 5] = s.qa[bare]
+        r[bare, 6] = s.qa[bare]
+        for j in (3, 5):
+            A[bare, 7, j] = 0.0
+            A[bare, 8, j] = 0.0
+        red = (fradu - s.pfluxu) * xu + (frads - s.pfluxs) * xs + (fradl - s.pfluxl) * xl
+        r[bare, 7] = r[bare, 7] + red[bare]
+
+    v = gauss_nopivot(A, r)
+    tun, tsn, tln, t12n, t34n, q12n, q34n, tgn, tin = (v[:, j] for j in range(9))
+
+    fsena = cp * cu * (s.ta * s.tfac - t12n)
+    fsenu = cp * su * (wu * tun + (1.0 - wu) * tu - t12n)
+    fsens = cp * ss * (ws * tsn + (1.0 - ws) * ts - t12n)
+    fsenl = cp * sl * (wl * tln + (1.0 - wl) * tl - t12n)
+    fseng = cp * sg * (wg * tgn + (1.0 - wg) * tg - t34n)
+    fseni = cp * si * (wi * tin + (1.0 - wi) * ti - t34n)
+    fvapa = cu * (s.qa - q12n)
+    quav = qu + wu * dqu * (tun - tu)
+    fvapuw = suw * (quav - q12n)
+    fvaput = np.maximum(0.0, sut * (quav - q12n))
+    qsav = qs + ws * dqs * (tsn - ts)
+    fvaps = ssw * (qsav - q12n)
+    qlav = ql + wl * dql * (tln - tl)
+    fvaplw = slw * (qlav - q34n)
+    fvaplt = np.maximum(0.0, slt0 * (qlav - q34n))
+    qgav = qg + wg * dqg * (tgn - tg)
+    fvapg = sg * (qgfac * qgav - q34n)
+    qiav = qi + wi * dqi * (tin - ti)
+    fvapi = si * (qiav - q34n)
+    firg = s.firg - wg * s.zirg * (tgn - s.tgold)
+    firi = s.firi - wi * s.ziri * (tin - s.tiold)
+    firb = (s.firb + (1.0 - fi) * wg * s.zirg * (tgn - s.tgold)
+            + fi * wi * s.ziri * (tin - s.tiold))
+    ginvap = fvapuw * xu + fvaps * xs + fvaplw * xl
+    gsuvap = fvapg * (1.0 - fi) + fvapi * fi
+    gtrans = fvaput * 2.0 * lai[:, 1] * s.fu + fvaplt * 2.0 * lai[:, 0] * s.fl * (1.0 - fi)
+    return SimpleNamespace(tu=tun, ts=tsn, tl=tln, t12=t12n, t34=t34n, q12=q12n, q34=q34n,
+                           tg=tgn, ti=tin, fsena=fsena, fsenu=fsenu, fsens=fsens, fsenl=fsenl,
+                           fseng=fseng, fseni=fseni, fvapa=fvapa, fvapuw=fvapuw, fvaput=fvaput,
+                           fvaps=fvaps, fvaplw=fvaplw, fvaplt=fvaplt, fvapg=fvapg, fvapi=fvapi,
+                           firb=firb, firg=firg, firi=firi, ginvap=ginvap, gsuvap=gsuvap,
+                           gtrans=gtrans, A=A, rhs=r)

@@ -1,0 +1,87 @@
+"""Field-by-field comparison against the oracle -- TEST INFRASTRUCTURE.
+
+Error metric per cell and field: ``|got - ref| / max(|ref|, floor)`` with ``floor`` = 1 % of
+the field's RMS over the cells (so a flux that happens to be near zero in one cell is
+judged against the field's own magnitude, not against zero).  Widths > 1 are compared per
+column.
+
+The delta band: a cell is ill-conditioned at scale ``rel`` if the oracle itself moves by
+more than the tolerance when every input is perturbed by a random relative ``rel`` (the
+rounding scale of the implementation under test).  Branches of the reference that sit on
+a threshold by construction (e.g. the snow-skin implicit weight of
+``canopy_exchange.py:448-463`` puts the blended temperature exactly on tmelt, where the
+ice and liquid branches of ``thermo.esat`` meet) flip on rounding alone, so no
+implementation can match them to the last bit; those cells are counted and reported
+separately, never silently passed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+INTEGER_LIKE = ()
+
+
+def rel_errors(got, ref):
+    g = np.asarray(got, dtype=np.float64)
+    r = np.asarray(ref, dtype=np.float64).reshape(g.shape)
+    if r.ndim == 1:
+        r2, g2 = r[:, None], g[:, None]
+    else:
+        r2, g2 = r, g
+    rms = np.sqrt(np.nanmean(np.where(np.isfinite(r2), r2 * r2, np.nan), axis=0))
+    rms = np.where(np.isfinite(rms), rms, 0.0)
+    floor = np.maximum(1e-2 * rms, 1e-300)
+    e = np.abs(g2 - r2) / np.maximum(np.abs(r2), floor[None, :])
+    e = np.where(np.isnan(g2) & np.isnan(r2), 0.0, e)
+    e = np.where(np.isnan(e), np.inf, e)
+    return e.max(axis=1)                      # per cell
+
+
+def compare(got: dict, ref: dict, keys, exclude=None):
+    """{key: (max, p99, n_over_1e-4)} over cells not in ``exclude``."""
+    out = {}
+    for k in keys:
+        e = rel_errors(got[k], ref[k])
+        if exclude is not None:
+            e = e[~exclude]
+        if e.size == 0:
+            out[k] = (0.0, 0.0, 0)
+            continue
+        out[k] = (float(e.max()), float(np.percentile(e, 99)), int((e > 1e-4).sum()))
+    return out
+
+
+def perturb(fields: dict, rel: float, seed: int = 1234, skip=()):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for k, v in fields.items():
+        a = np.asarray(v, dtype=np.float64)
+        if k in skip:
+            out[k] = a
+        else:
+            out[k] = a * (1.0 + rel * rng.standard_normal(a.shape))
+    return out
+
+
+#: static soil and vegetation properties: passed bit-identically to every implementation,
+#: so they are not perturbed (perturbing b = 4.5 would flip the F77 nint of
+#: soil_physics.py:249-252 for every loam cell)
+STATIC_KEYS = ("soil.csoi", "soil.rhosoi", "soil.z0soi", "soil.poros", "soil.hydraul",
+               "soil.suction", "soil.bex", "soil.sfield", "soil.swilt", "soil.sand", "soil.clay",
+               "soil.albsav", "soil.albsan", "vegetation.frac", "vegetation.exist",
+               "vegetation.lai", "vegetation.sai", "vegetation.ztop", "vegetation.zbot",
+               "vegetation.fu", "vegetation.fl")
+
+
+def delta_band(run, fields: dict, keys, rel: float, tol: float, seed: int = 1234):
+    """Mask of cells whose oracle output moves by more than ``tol`` when the prognostic
+    state and forcing are perturbed by a random relative ``rel``; ``run(fields) -> dict``."""
+    ref = run(fields)
+    alt = run(perturb(fields, rel, seed, skip=STATIC_KEYS))
+    bad = None
+    for k in keys:
+        e = rel_errors(alt[k], ref[k]) > tol
+        bad = e if bad is None else (bad | e)
+    if "_knife_edge" in ref:
+        bad = bad | np.asarray(ref["_knife_edge"], dtype=bool)
+    return bad, ref

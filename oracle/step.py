@@ -1,0 +1,398 @@
+"""The hourly step ``timestep_body`` (FP64), restating ``python-port/ibis/timestepping/loops.py:234-1284``.
+
+``timestep_body(st, consts, params, dtime, glob)`` maps a flat dict of reference-shaped
+arrays ``{"soil.tsoi": (N, 6), "snow.fi": (N,), ...}`` (the fields of
+``ibis_amd.layout.FIELDS`` with role ``in``/``io``) to a dict of every ``io``/``out`` field.
+``glob`` carries the leaves that are not per cell: ``hsoi`` (7,), ``froot`` (6, 2),
+``co2conc``, ``nytimes`` and the snow constants.  The F77 path is restated (all optional
+StomataParams switches off; ``h2o_consistent_weights`` honoured).
+
+Pinned, as a whole, by: column water balance closure (``tests/test_water_balance.py:35-47``);
+no negative puddle on a dry surface (``tests/test_puddle_conservation.py:95-186``); night
+steps with zero absorbed solar (``tests/test_integration.py:239-250``); bare cells taking
+the atmospheric t/q (``canopy_exchange.py:709-747``); and finiteness/bounds over 48 steps
+(``tests/test_regression.py:158-216``).  The ordering of the three canopy passes and of the
+snow glue (``loops.py:473-611``, ``:729-848``) has no reference-held vector: parity
+unpinned beyond those invariants.
+"""
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import canopy as can_
+from . import fire as fire_
+from . import interception as icp
+from . import radiation as rad_
+from . import snow as snow_
+from . import soil as soil_
+from . import stomata as stom_
+from ._num import ipow
+
+FLUX_MAX = 2000.0       # loops.py:665-666 (IBIS_FLUX_MAX default)
+T_MAX_DEV = 40.0        # loops.py:1143-1144 (IBIS_T_MAX_DEV default)
+FVAPG_PHYS_MAX = 1.0e-3  # loops.py:875
+
+
+def _solid_phase(st, p):
+    """Solid-phase specific heat and density, state first, constant fallback (loops.py:190-215)."""
+    cs = st["soil.csoi"]
+    rh = st["soil.rhosoi"]
+    return (np.where(cs > 0.0, cs, p.csoi_mineral_val), np.where(rh > 0.0, rh, p.rhosoi_val))
+
+
+def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV):
+    g = lambda k: np.asarray(st[k], dtype=np.float64)
+    ip, cp_, sp = p.interception, p.canopy, p.stomata_params
+    sn = glob.snow
+    eps, tmelt = c.epsilon, c.tmelt
+    hsoi = np.asarray(glob.hsoi, dtype=np.float64)
+    froot = np.asarray(glob.froot, dtype=np.float64)
+    ns = 6
+
+    tsoi, wsoi, wisoi = g("soil.tsoi"), g("soil.wsoi"), g("soil.wisoi")
+    tg0, wpud, wipud = g("soil.tg"), g("soil.wpud"), g("soil.wipud")
+    poros = g("soil.poros")
+    fi0, tsno, hsno, ti0, tlsub = (g("snow.fi"), g("snow.tsno"), g("snow.hsno"), g("snow.ti"),
+                                   g("snow.tlsub"))
+    tu0, ts0, tl0 = g("canopy.tu"), g("canopy.ts"), g("canopy.tl")
+    ta, qa, ua, psurf = g("atm.ta"), g("atm.qa"), g("atm.ua"), g("atm.psurf")
+    coszen, solad, solai = g("atm.coszen"), g("atm.solad"), g("atm.solai")
+    fira, raina, snowa = g("atm.fira"), g("atm.raina"), g("atm.snowa")
+    lai, sai, fu, fl = g("vegetation.lai"), g("vegetation.sai"), g("vegetation.fu"), g("vegetation.fl")
+    frac, exist = g("vegetation.frac"), g("vegetation.exist")
+    stores = [g("canopy." + k) for k in ("wliqu", "wliqs", "wliql", "wsnou", "wsnos", "wsnol")]
+
+    # 1. diagnostic soil properties (loops.py:318-326)
+    ss = soil_.setsoi(g("soil.sand"), g("soil.clay"), poros, wsoi, wisoi, tsoi, wpud, wipud,
+                      p.wpudmax, p.zwpmax, tsno[:, 0], ta, c)
+    # 2. wet fractions before the cascade (loops.py:331-338)
+    fw = icp.fwetcal(tu0, ts0, tl0, *stores, ip, c)
+    # 3. radiation (loops.py:343-354)
+    rad = rad_.compute_radiation(coszen, solad, solai, fira, lai, sai, fu, fl, tu0, ts0, tl0,
+                                 tg0, ti0, fi0, tsno[:, 0], wsoi[:, 0], wisoi[:, 0],
+                                 g("soil.albsav"), g("soil.albsan"), fw, p.veg_rad, c)
+    # 4. precipitation cascade (loops.py:359-369)
+    cas = icp.cascade(tu0, ts0, tl0, g("canopy.t12"), g("canopy.t34"), *stores, lai, sai, fu,
+                      fl, ta, qa, psurf, raina, snowa, ip, dtime, c)
+    wliqu, wliqs, wliql = cas.wliqu, cas.wliqs, cas.wliql
+    wsnou, wsnos, wsnol = cas.wsnou, cas.wsnos, cas.wsnol
+    # 5. wet fractions after the cascade (loops.py:382-389)
+    fw2 = icp.fwetcal(tu0, ts0, tl0, wliqu, wliqs, wliql, wsnou, wsnos, wsnol, ip, c)
+    # 6a. aerodynamics (loops.py:396-402)
+    aero = can_.canini(ta, qa, psurf, ua, fi0, g("soil.z0soi"), sn.z0sno, fl, fu, lai, sai,
+                       g("vegetation.ztop"), g("vegetation.zbot"), cp_, c)
+    # 6b. soil-water stress (loops.py:416-419)
+    stresstl, stresstu, stressl, stressu = stom_.drystress(wsoi, wisoi, g("soil.swilt"),
+                                                           g("soil.sfield"), froot, sp.stressfac)
+    # 6c. intermediates (loops.py:147-183, 422-432)
+    csoi, rhosoi = _solid_phase(st, p)
+    xu = 2.0 * lai[:, 1] * fu
+    xs = 2.0 * sai[:, 1] * fu
+    xl = 2.0 * (lai[:, 0] + sai[:, 0]) * fl * (1.0 - fi0)
+    cog = ss.consoi[:, 0] / max(0.5 * hsoi[0], eps)
+    coi = sn.consno / np.maximum(0.5 * np.maximum(hsno[:, 0], sn.hsnotop), eps)
+    zirg = 4.0 * 0.95 * c.stef * ipow(tg0, 3)
+    ziri = 4.0 * 0.95 * c.stef * ipow(ti0, 3)
+    rwp = poros[:, 0] * c.rhow
+    chgx = (c.ch2o * wpud + c.cice * wipud
+            + ((1.0 - poros[:, 0]) * csoi[:, 0] * rhosoi[:, 0]
+               + rwp * (1.0 - wisoi[:, 0]) * wsoi[:, 0] * c.ch2o
+               + rwp * wisoi[:, 0] * c.cice) * hsoi[0])
+    wlgx = wpud + rwp * (1.0 - wisoi[:, 0]) * wsoi[:, 0] * hsoi[0]
+    wigx = wipud + rwp * wisoi[:, 0] * hsoi[0]
+
+    # 6d. three fixed passes (loops.py:473-602)
+    tu, ts, tl = tu0, ts0, tl0
+    t12, t34, q12, q34 = g("canopy.t12"), g("canopy.t34"), g("canopy.q12"), g("canopy.q34")
+    tg, ti = tg0, ti0
+    n = tu.shape[0]
+    ones = np.ones(n)
+    carry = SimpleNamespace(straml=ones, strahl=ones, stramu=ones, strahu=ones)
+    wprev = [ones] * 5
+    ci_u, gs_u, cs_u = g("canopy.ci_upper"), g("canopy.gs_upper"), g("canopy.cs_upper")
+    ci_l, gs_l, cs_l = g("canopy.ci_lower"), g("canopy.gs_lower"), g("canopy.cs_lower")
+    a10 = (g("canopy.a10scalparamu"), g("canopy.a10daylightu"), g("canopy.a10scalparaml"),
+           g("canopy.a10daylightl"))
+    u2c, aloguc, aloglc = aero.u2, aero.alogu, aero.alogl
+    res = stom = None
+    # cells whose snow-skin implicit weight is fractional in some pass: the blended ti is then
+    # tmelt up to rounding, where esat/desat switch branch (structural knife edge)
+    knife = np.zeros(tg0.shape, dtype=bool)
+    chux = cp_.chu + c.ch2o * wliqu + c.cice * wsnou
+    chsx = cp_.chs + c.ch2o * wliqs + c.cice * wsnos
+    chlx = cp_.chl + c.ch2o * wliql + c.cice * wsnol
+    for it in range(3):
+        first = it == 0
+        a_it = SimpleNamespace(**vars(aero))
+        a_it.u2, a_it.alogu, a_it.alogl = u2c, aloguc, aloglc
+        tc = can_.turcof(a_it, ta, qa, ua, t12, t34, q12, q34, c, cp_, dtime, first, carry)
+        stom = stom_.stomata(tu, tl, t12, t34, q12, q34, tc.su, tc.sl, psurf, rad, ci_u, gs_u,
+                             cs_u, ci_l, gs_l, cs_l, fw2.fwetu, fw2.fwetl, lai, sai, exist,
+                             stresstu, stresstl, a10, glob.co2conc, dtime, sp, eps)
+        ci_u, gs_u, cs_u = stom.ci_upper, stom.gs_upper, stom.cs_upper
+        ci_l, gs_l, cs_l = stom.ci_lower, stom.gs_lower, stom.cs_lower
+        a10 = stom.a10
+        wu = can_.impexp(tu, chux, wliqu, wsnou, c, wprev[0], first)
+        ws = can_.impexp(ts, chsx, wliqs, wsnos, c, wprev[1], first)
+        wl = can_.impexp(tl, chlx, wliql, wsnol, c, wprev[2], first)
+        wg = can_.impexp(tg, chgx, wlgx, wigx, c, wprev[3], first)
+        wi = can_.impexp2(ti, ti0, c, wprev[4], first)
+        knife = knife | ((wi > 0.0) & (wi < 1.0))
+        s = SimpleNamespace(
+            tu=tu, ts=ts, tl=tl, t12=t12, t34=t34, q12=q12, q34=q34, tg=tg, ti=ti,
+            tuold=tu0, tsold=ts0, tlold=tl0, tgold=tg0, tiold=ti0,
+            cu=tc.cu, cl=tc.cl, su=tc.su, ss=tc.ss, sl=tc.sl, sg=tc.sg, si=tc.si,
+            cp=aero.cp, rhoa=aero.rhoa, tfac=aero.tfac, xu=xu, xs=xs, xl=xl, fi=fi0, fu=fu,
+            fl=fl, lai=lai, sai=sai, frac=frac, chux=chux, chsx=chsx, chlx=chlx,
+            wliqu=wliqu, wliqs=wliqs, wliql=wliql, wsnou=wsnou, wsnos=wsnos, wsnol=wsnol,
+            rliqu=fw2.rliqu, rliqs=fw2.rliqs, rliql=fw2.rliql,
+            fwetu=fw2.fwetu, fwets=fw2.fwets, fwetl=fw2.fwetl,
+            totcondub=stom.totcondub, totconduc=stom.totconduc, totcondls=stom.totcondls,
+            totcondl3=stom.totcondl3, totcondl4=stom.totcondl4,
+            cog=cog, coi=coi, zirg=zirg, ziri=ziri,
+            wpud=wpud, wipud=wipud, wpudmax=p.wpudmax, wisoi1=wisoi[:, 0], wsoi1=wsoi[:, 0],
+            swilt1=g("soil.swilt")[:, 0], suction1=g("soil.suction")[:, 0],
+            bex1=g("soil.bex")[:, 0], hvasug=ss.hvasug, hvasui=ss.hvasui,
+            tsoi1=tsoi[:, 0], tsno1=tsno[:, 0],
+            solu=rad.solu, sols=rad.sols, soll=rad.soll, solg=rad.solg, soli=rad.soli,
+            firu=rad.firu, firs=rad.firs, firl=rad.firl, firg=rad.firg, firi=rad.firi,
+            firb=rad.firb, pfluxu=cas.pfluxu, pfluxs=cas.pfluxs, pfluxl=cas.pfluxl,
+            ta=ta, qa=qa, psurf=psurf, wu=wu, ws=ws, wl=wl, wg=wg, wi=wi)
+        res = can_.turvap(s, c, dtime, beta_power=cp_.soil_beta_power)
+        tu, ts, tl, t12, t34 = res.tu, res.ts, res.tl, res.t12, res.t34
+        q12, q34, tg, ti = res.q12, res.q34, res.tg, res.ti
+        carry = SimpleNamespace(straml=tc.straml, strahl=tc.strahl, stramu=tc.stramu,
+                                strahu=tc.strahu)
+        u2c, aloguc, aloglc = tc.u2, tc.alogu, tc.alogl
+        wprev = [wu, ws, wl, wg, wi]
+    # ti clamp after the last pass only (loops.py:604-611)
+    ti = np.minimum(res.ti, tmelt)
+
+    # 7. evaporation cascade (loops.py:616-627)
+    (tu, ts, tl, wliqu2, wliqs2, wliql2, wsnou2, wsnos2, wsnol2, fvapa2, fsena2) = icp.cascad2(
+        tu, ts, tl, wliqu, wliqs, wliql, wsnou, wsnos, wsnol, fw2.rliqu, fw2.rliqs, fw2.rliql,
+        res.fvapuw, res.fvaps, res.fvaplw, fu, fl, fi0, lai, sai, cp_, dtime, ta, res.fvapa,
+        res.fsena, c)
+    # 8. no-vegetation reset (loops.py:633-640)
+    (tu_nv, ts_nv, tl_nv, wliqu_nv, wliqs_nv, wliql_nv, wsnou_nv, wsnos_nv, wsnol_nv) = icp.noveg(
+        tu, ts, tl, wliqu2, wliqs2, wliql2, wsnou2, wsnos2, wsnol2, lai, sai, fu, fl, fi0, tg, ti)
+
+    # 9. surface heat fluxes with the +-FLUX_MAX clamp (loops.py:665-674)
+    firg_c = np.clip(res.firg, -flux_max, flux_max)
+    fseng_c = np.clip(res.fseng, -flux_max, flux_max)
+    firi_c = np.clip(res.firi, -flux_max, flux_max)
+    fseni_c = np.clip(res.fseni, -flux_max, flux_max)
+    heatg = rad.solg + firg_c - fseng_c - ss.hvasug * res.fvapg
+    heati = rad.soli + firi_c - fseni_c - ss.hvasui * res.fvapi
+
+    # 10. snow (loops.py:679-848)
+    rhos = sn.rhos
+    fhtop_snow = (heati + cas.rainl * (c.ch2o * (cas.trainl - tmelt) + c.hfus
+                                       + c.cice * (tmelt - tsno[:, 0]))
+                  + cas.snowl * c.cice * (cas.tsnowl - tsno[:, 0]))
+    xlb = fl * 2.0 * (lai[:, 0] + sai[:, 0])
+    hs1 = hsno.copy()
+    hs1[:, 0] = hsno[:, 0] + (cas.rainl + cas.snowl - res.fvapi) * (dtime / rhos)
+    hs1 = np.where(fi0[:, None] > 0.0, hs1, hsno)
+    tsn_h, tlsub_h, sflo = snow_.snowheat(tsno, tlsub, fhtop_snow, xlb, cp_.chl, fi0, hs1, rhos,
+                                          sn.consno, hsoi[0], ss.consoi[:, 0], tsoi[:, 0], dtime,
+                                          c.cice)
+    ht = hs1[:, 0] + hs1[:, 1] + hs1[:, 2]
+    hts = np.where(ht > 0.0, ht, sn.hsnomin)
+    dfi = (1.0 - fi0) * dtime * cas.snowg / (rhos * hts)
+    dfi = np.minimum(dfi, 1.0 - fi0)
+    fi1 = fi0 + dfi
+    tsn_m = np.where((fi1 > 0.0)[:, None],
+                     (tsn_h * fi0[:, None] + cas.tsnowg[:, None] * dfi[:, None])
+                     / np.maximum(fi1[:, None], eps), tsn_h)
+    hinit = np.full_like(hs1, (sn.hsnomin - sn.hsnotop) / max(3 - 1, 1))
+    hinit[:, 0] = sn.hsnotop
+    hs2 = np.where(((fi0 < eps) & (dfi > 0.0))[:, None], hinit, hs1)
+    hs3, tsn3, smelt, zheat = snow_.snow_melt(hs2, tsn_m, fi1, rhos, c.cice, c.hfus, tmelt)
+    hsnoruf = np.minimum(0.70, np.maximum(sn.hsnomin + 0.05, fl * g("vegetation.ztop")[:, 0]))
+    fi2, hs4 = snow_.snow_coverage(hs3, fi1, sn.hsnomin, hsnoruf, sn.fimax, eps)
+    hs5, tsn5 = snow_.vadapt(hs4, tsn3, sn.hsnotop, (hs4[:, 0] + hs4[:, 1] + hs4[:, 2]) > eps,
+                             eps)
+    small = fi2 < sn.fimin
+    mass = rhos * (hs5[:, 0] + hs5[:, 1] + hs5[:, 2]) * fi2
+    hq = rhos * hs5 * fi2[:, None] * (c.cice * (tmelt - tsn5) + c.hfus)
+    heat_rem = -(hq[:, 0] + hq[:, 1] + hq[:, 2])
+    smelt = np.where(small, smelt + mass, smelt)
+    zheat = np.where(small, zheat + heat_rem, zheat)
+    hs6 = np.where(small[:, None], 0.0, hs5)
+    tsn6 = np.where(small[:, None], tmelt, tsn5)
+    fi_new = np.where(small, 0.0, fi2)
+    dfc = fi_new - fi0
+    tlsub_o = np.where(dfc > 0, (tlsub_h * fi0 + tl_nv * dfc) / np.maximum(fi_new, eps), tlsub_h)
+    zheat = np.where(dfc > 0, zheat + dfc * xlb * (
+        wliql_nv * (c.ch2o * (tl_nv - tmelt) + c.hfus + c.cice * (tmelt - tsn6[:, -1]))
+        + wsnol_nv * c.cice * (tl_nv - tsn6[:, -1])), zheat)
+    hs7 = hs6.copy()
+    hs7[:, -1] = hs6[:, -1] + dfc * xlb * (wliql_nv + wsnol_nv) / (rhos * np.maximum(fi_new, eps))
+    hs7 = np.where((dfc > 0)[:, None], hs7, hs6)
+    zheat = np.where(dfc < 0, zheat - dfc * xlb * cp_.chl * (tlsub_h - tl_nv), zheat)
+    rdec = np.where(dfc < 0, (1.0 - fi0) / np.maximum(1.0 - fi_new, eps), 1.0)
+    wliql_nv = wliql_nv * rdec
+    wsnol_nv = wsnol_nv * rdec
+    tlsub_o = np.where(fi_new > 0, tlsub_o, tl_nv)
+
+    # areal averages (loops.py:850-894)
+    rdt = 1.0 / dtime
+    heatg_avg = (1.0 - fi0) * heatg + fi0 * sflo[:, -1] + zheat * rdt
+    fvapg_avg = np.clip((1.0 - fi0) * res.fvapg, -FVAPG_PHYS_MAX, FVAPG_PHYS_MAX)
+    raing_nf = (1.0 - fi0) * cas.raing
+    smelt_rate = smelt * rdt
+    r_un = raing_nf + smelt_rate
+    rzero = r_un < eps
+    raing_avg = np.where(rzero, 0.0, r_un)
+    traing_avg = np.where(rzero, 0.0, (raing_nf * cas.traing + smelt_rate * tmelt)
+                          / np.maximum(r_un, eps))
+
+    # 11. soil (loops.py:896-1125)
+    c0pud = np.zeros_like(tsoi)
+    c0pud[:, 0] = wpud * c.ch2o + wipud * c.cice
+    wp, wip = wpud, wipud
+    grunof = np.minimum(wp, np.maximum(0.0, wp + wip - p.wpudmax)) / dtime
+    wp = wp - grunof * dtime
+    zfrez = np.clip((tsoi[:, 0] - tmelt + 0.5) * 2.0, 0.0, 1.0)
+    zrunf = zfrez * np.clip((wp + wip) / p.wpudmax, 0.0, 1.0)
+    wp = wp + (1.0 - zrunf) * raing_avg * dtime
+    grunof = grunof + zrunf * raing_avg
+    q = ss.qglif
+    evap = fvapg_avg >= 0.0
+    edt = fvapg_avg * dtime
+    fwtop = np.where(evap, -q[:, 0] * fvapg_avg, 0.0)
+    wp = np.where(evap, wp - q[:, 2] * edt, wp)
+    wip = np.where(evap, wip - q[:, 3] * edt, wip)
+    isub = np.where(evap, q[:, 1] * edt / np.maximum(c.rhow * poros[:, 0] * hsoi[0], eps), 0.0)
+    wisoi0n = np.maximum(0.0, wisoi[:, 0] - isub)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        wsoi0a = np.where((1.0 - wisoi0n) > eps, wsoi[:, 0] * (1.0 - wisoi[:, 0]) / (1.0 - wisoi0n),
+                          wsoi[:, 0])
+    wp = np.where(~evap, wp - (q[:, 0] + q[:, 2]) * edt, wp)
+    wip = np.where(~evap, wip - (q[:, 1] + q[:, 3]) * edt, wip)
+    wp = np.maximum(wp, 0.0)
+    wip = np.maximum(wip, 0.0)
+    zdpud = c.rhow * dtime * (np.maximum(1.0 - wisoi0n, 0.0) ** 2) * g("soil.hydraul")[:, 0]
+    fwpud = np.maximum(0.0, np.minimum(wp, zdpud)) / dtime
+    c0pud[:, 0] = c.ch2o * wp + c.cice * wip
+    wsoi_w = wsoi.copy()
+    wisoi_w = wisoi.copy()
+    wsoi_w[:, 0] = np.where(evap, wsoi0a, wsoi[:, 0])
+    wisoi_w[:, 0] = np.where(evap, wisoi0n, wisoi[:, 0])
+    porosflo = np.maximum(poros * np.maximum(1.0 - wisoi_w, eps), eps)
+
+    gtransu = res.fvaput * 2.0 * lai[:, 1] * fu
+    gtransl = res.fvaplt * 2.0 * lai[:, 0] * fl * (1.0 - fi0)
+    sstu = np.maximum(stresstu, eps)
+    sstl = np.maximum(stresstl, eps)
+    weightu = np.where(stresstu[:, None] >= 1.0, froot[None, :, 1], stressu / sstu[:, None])
+    weightl = np.where(stresstl[:, None] >= 1.0, froot[None, :, 0], stressl / sstl[:, None])
+    wsoi_t = wsoi_w.copy()
+    tsoi_t = tsoi.copy()
+    for k in range(ns):
+        upu = gtransu * weightu[:, k]
+        upl = gtransl * weightl[:, k]
+        wsoi_t[:, k] = wsoi_t[:, k] + (-dtime * (upu + upl) / (c.rhow * porosflo[:, k] * hsoi[k]))
+        wsoi_t[:, k] = np.maximum(wsoi_t[:, k], 0.0)
+        cx = (c0pud[:, k] + ((1.0 - poros[:, k]) * csoi[:, k] * rhosoi[:, k]
+                             + poros[:, k] * (1.0 - wisoi_w[:, k]) * wsoi_t[:, k] * c.ch2o * c.rhow
+                             + poros[:, k] * wisoi_w[:, k] * c.cice * c.rhow) * hsoi[k])
+        tsoi_t[:, k] = tsoi_t[:, k] + (-dtime * c.ch2o * (upu * (tu_nv - tsoi_t[:, k])
+                                                         + upl * (tl_nv - tsoi_t[:, k]))
+                                       / np.maximum(cx, eps))
+    otsoi, owsoi = tsoi_t, wsoi_t
+    fhtop_soil = (heatg_avg + raing_avg * c.ch2o * (traing_avg - otsoi[:, 0])
+                  - grunof * c.ch2o * np.maximum(tmelt - otsoi[:, 0], 0.0))
+    h2o = soil_.soilh2o(wsoi_t, wisoi_w, poros, porosflo, g("soil.hydraul"), g("soil.suction"),
+                        g("soil.bex"), hsoi[:ns], fwtop, fwpud, wp, p.bperm, dtime, c.rhow, eps,
+                        consistent_weights=bool(p.h2o_consistent_weights))
+    c1pud = c0pud.copy()
+    c1pud[:, 0] = c.ch2o * h2o.wpud + c.cice * wip
+    if "soil.tsoi_deep" in st and getattr(p, "bedrock", None) is not None:
+        td = g("soil.tsoi_deep")
+        tsoi_h, hflo, tdeep_n = soil_.soilheat_with_bedrock(
+            tsoi_t, td, otsoi, td, owsoi, h2o.wsoi, wisoi_w, poros, ss.consoi, csoi, rhosoi,
+            hsoi[:ns], c0pud, c1pud, fhtop_soil, h2o.wflo, c.ch2o, c.cice, c.rhow, dtime,
+            p.bedrock)
+    else:
+        tsoi_h, hflo = soil_.soilheat(tsoi_t, otsoi, owsoi, h2o.wsoi, wisoi_w, poros, ss.consoi,
+                                      csoi, rhosoi, hsoi[:ns], c0pud, c1pud, fhtop_soil,
+                                      h2o.wflo, c.ch2o, c.cice, c.rhow, dtime)
+        tdeep_n = None
+    w_a, wi_a, gadj1 = soil_.wadjust(h2o.wsoi, wisoi_w, poros, hsoi[:ns], c.rhow, dtime, eps)
+    tsoi_f, wsoi_f, wisoi_f, porosflo_f = soil_.freeze_thaw(tsoi_h, w_a, wi_a, poros, csoi,
+                                                            rhosoi, hsoi[:ns], c1pud, c)
+    t0f, wpud_f, wipud_f = soil_.puddle_freeze_thaw(tsoi_f[:, 0], h2o.wpud, wip, poros[:, 0],
+                                                    wsoi_f[:, 0], wisoi_f[:, 0], csoi[:, 0],
+                                                    rhosoi[:, 0], hsoi[0], c)
+    tsoi_f[:, 0] = t0f
+    wsoi_o, wisoi_o, gadj2 = soil_.wadjust(wsoi_f, wisoi_f, poros, hsoi[:ns], c.rhow, dtime, eps)
+
+    # 12. surface temperature bound (loops.py:1143-1151)
+    lo, hi = ta - t_max_dev, ta + t_max_dev
+    tu_nv, ts_nv, tl_nv = np.clip(tu_nv, lo, hi), np.clip(ts_nv, lo, hi), np.clip(tl_nv, lo, hi)
+    tg = np.clip(tg, lo, hi)
+
+    # 13. fire accumulators (loops.py:1246-1275)
+    nyt = float(glob.nytimes) + 1.0
+    swilt1 = g("soil.swilt")[:, 0]
+    firefac = fire_.update_firefac(g("vegetation.firefac"), nyt, wsoi_o[:, 0], wisoi_o[:, 0],
+                                   swilt1)
+    nfs, sdry = fire_.update_accumulators(g("vegetation.nfirestep"), g("vegetation.sdry"),
+                                          wsoi_o[:, 0], wisoi_o[:, 0], swilt1, p.fire.d_crit)
+    nfv, sdv = fire_.update_accumulators_vpd(g("vegetation.nfirestep_vpd"),
+                                             g("vegetation.sdry_vpd"), ta, qa, psurf, p.fire)
+
+    out = {
+        "soil.tsoi": tsoi_f, "soil.wsoi": wsoi_o, "soil.wisoi": wisoi_o, "soil.tg": tg,
+        "soil.wpud": wpud_f, "soil.wipud": wipud_f, "soil.consoi": ss.consoi,
+        "soil.hflo": hflo, "soil.qglif": ss.qglif, "soil.heatg": heatg_avg,
+        "soil.hvasug": ss.hvasug, "soil.stresstl": stresstl, "soil.stresstu": stresstu,
+        "soil.stressl": stressl, "soil.stressu": stressu, "soil.porosflo": porosflo_f,
+        "snow.fi": fi_new, "snow.tsno": tsn6, "snow.hsno": hs7, "snow.ti": ti,
+        "snow.tlsub": tlsub_o, "snow.heati": heati, "snow.hvasui": ss.hvasui,
+        "canopy.tu": tu_nv, "canopy.ts": ts_nv, "canopy.tl": tl_nv, "canopy.t12": t12,
+        "canopy.t34": t34, "canopy.q12": q12, "canopy.q34": q34,
+        "canopy.wliqu": wliqu_nv, "canopy.wliqs": wliqs_nv, "canopy.wliql": wliql_nv,
+        "canopy.wsnou": wsnou_nv, "canopy.wsnos": wsnos_nv, "canopy.wsnol": wsnol_nv,
+        "canopy.ci_upper": stom.ci_upper, "canopy.gs_upper": stom.gs_upper,
+        "canopy.cs_upper": stom.cs_upper, "canopy.an_upper": stom.an_upper,
+        "canopy.ag_upper": stom.ag_upper, "canopy.ci_lower": stom.ci_lower,
+        "canopy.gs_lower": stom.gs_lower, "canopy.cs_lower": stom.cs_lower,
+        "canopy.an_lower": stom.an_lower, "canopy.ag_lower": stom.ag_lower,
+        "canopy.fwetu": fw2.fwetu, "canopy.fwets": fw2.fwets, "canopy.fwetl": fw2.fwetl,
+        "canopy.rliqu": fw2.rliqu, "canopy.rliqs": fw2.rliqs, "canopy.rliql": fw2.rliql,
+        "canopy.solu": rad.solu, "canopy.sols": rad.sols, "canopy.soll": rad.soll,
+        "canopy.solg": rad.solg, "canopy.soli": rad.soli,
+        "canopy.firu": rad.firu, "canopy.firs": rad.firs, "canopy.firl": rad.firl,
+        "canopy.firg": res.firg, "canopy.firi": res.firi, "canopy.firb": res.firb,
+        "canopy.asurd": rad.asurd, "canopy.asuri": rad.asuri,
+        "canopy.topparu": rad.topparu, "canopy.topparl": rad.topparl,
+        "canopy.scalcoefu": rad.scalcoefu, "canopy.scalcoefl": rad.scalcoefl,
+        "canopy.fsenu": res.fsenu, "canopy.fsens": res.fsens, "canopy.fsenl": res.fsenl,
+        "canopy.fseng": res.fseng, "canopy.fseni": res.fseni, "canopy.fvapg": res.fvapg,
+        "canopy.fvapi": res.fvapi, "canopy.raing": cas.raing, "canopy.snowg": cas.snowg,
+        "canopy.a10scalparamu": stom.a10[0], "canopy.a10daylightu": stom.a10[1],
+        "canopy.a10scalparaml": stom.a10[2], "canopy.a10daylightl": stom.a10[3],
+        "canopy.rhoa": aero.rhoa,
+        "vegetation.firefac": firefac, "vegetation.nfirestep": nfs, "vegetation.sdry": sdry,
+        "vegetation.nfirestep_vpd": nfv, "vegetation.sdry_vpd": sdv,
+        "hydrology.ginvap": res.ginvap, "hydrology.gsuvap": res.gsuvap,
+        "hydrology.gtrans": res.gtrans, "hydrology.gtransu": gtransu,
+        "hydrology.gtransl": gtransl, "hydrology.grunof": grunof,
+        "hydrology.gdrain": h2o.wflo[:, -1], "hydrology.gadjust": gadj1 + gadj2,
+        "hydrology.wtot": np.zeros(n), "hydrology.raing_avg": raing_avg,
+        "hydrology.traing_avg": traing_avg, "hydrology.fhtop_soil": fhtop_soil,
+        "hydrology.rainl": cas.rainl, "hydrology.snowl": cas.snowl,
+        "hydrology.trainl": cas.trainl, "hydrology.tsnowl": cas.tsnowl,
+        "hydrology.sflo_bot": sflo[:, -1], "hydrology.fhtop_snow": fhtop_snow,
+        "diag.fsena": fsena2, "diag.fvapa": fvapa2,
+    }
+    if tdeep_n is not None:
+        out["soil.tsoi_deep"] = tdeep_n
+    out["_nytimes"] = nyt
+    out["_knife_edge"] = knife
+    return out

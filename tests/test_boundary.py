@@ -1,0 +1,109 @@
+"""The C ABI and its host-side mirror: layout, exported symbols, argument checks, defaults."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ibis_amd import abi, layout as L, params as PP, synth
+from ibis_amd import _lib
+from oracle import params as OP
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_layout_header_is_current():
+    with open(os.path.join(ROOT, "include", "ibis_layout.h")) as fh:
+        assert fh.read() == L.render_header()
+
+
+def test_io_contract_matches_the_reference_leaf_count():
+    # 179 fp32 words read, 180 written per cell (+12 int8 slot ids) = 1,448 B per cell-hour
+    assert L.words_read() == 179
+    assert L.words_written() == 180
+    assert L.ALGORITHMIC_BYTES_PER_CELL_HOUR == 1448
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    with open(os.path.join(ROOT, "include", "ibis_step.h")) as fh:
+        decl = re.findall(r"^(?:int|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
+    assert set(decl) == set(_lib.EXPORTS)
+    for name in decl:
+        assert hasattr(lib, name)
+    assert b"gfx950" in lib.ibis_version()
+
+
+def test_params_struct_matches_the_c_layout():
+    # offsets of the C struct (include/ibis_step.h) as the compiler lays it out
+    assert C.sizeof(abi.IbisParams) % 8 == 0
+    assert abi.IbisParams.co2conc.offset == C.sizeof(abi.IbisParams) - 8
+
+
+def _call(lib, ain, aout, n, flags, p=None, table=1, dtime=3600.0):
+    p = p if p is not None else abi.IbisParams()
+    return lib.ibis_step(C.byref(ain) if ain else None, C.byref(aout) if aout else None, None,
+                         C.c_int64(0), C.c_void_p(table), C.byref(p), C.c_double(dtime),
+                         C.c_double(0.0), C.c_int64(n), C.c_int(flags), None)
+
+
+def test_argument_errors_return_codes_without_touching_the_gpu():
+    lib = _lib.load()
+    a = abi.IbisArena(1, 10)
+    assert _call(lib, None, a, 4, 0) == -1
+    assert _call(lib, a, a, -1, 0) == -1
+    assert _call(lib, a, a, 20, 0) == -1                  # pitch < ncell
+    assert _call(lib, a, a, 4, 0x100) == -2               # unknown flag
+    assert _call(lib, a, a, 4, 3) == -2                   # unknown precision
+    assert _call(lib, a, a, 4, 0, dtime=0.0) == -1
+    p = abi.IbisParams()
+    assert _call(lib, a, a, 4, abi.FLAG_BEDROCK, p=p) == -4   # bedrock thicknesses unset
+    assert b"bedrock" in lib.ibis_last_error()
+    assert _call(lib, a, a, 0, 0) == 0                    # nothing to do
+
+
+def test_product_defaults_equal_the_reference_defaults():
+    c, oc = PP.get_constants(), OP.physical_constants()
+    for k in vars(oc):
+        assert getattr(c, k) == getattr(oc, k), k
+    sp, osp = PP.SimParams().stomata_params, OP.stomata_params()
+    for k, v in vars(osp).items():
+        np.testing.assert_array_equal(np.asarray(getattr(sp, k)), np.asarray(v), err_msg=k)
+    for a, b in ((PP.CanopyParams(), OP.canopy_params()), (PP.InterceptionParams(), OP.interception_params()),
+                 (PP.BedrockParams(), OP.bedrock_params()), (PP.FireParams(), OP.fire_params()),
+                 (PP.SnowConstants(), OP.snow_constants())):
+        for k, v in vars(b).items():
+            assert np.all(np.asarray(getattr(a, k)) == np.asarray(v)), k
+    vr, ovr = PP.VegRadParams(), OP.veg_rad_params()
+    for k, v in vars(ovr).items():
+        np.testing.assert_array_equal(getattr(vr, k), v)
+    s = PP.SimParams()
+    for k in ("wpudmax", "zwpmax", "bperm", "rhosoi_val", "csoi_mineral_val"):
+        assert getattr(s, k) == getattr(OP.sim_params(), k)
+
+
+def test_type_table_gathers_the_parent_pft():
+    from oracle.run import gather_vmax, TYPE_PARENT
+    np.testing.assert_array_equal(synth.TYPE_PARENT, TYPE_PARENT)
+    rng = np.random.default_rng(0)
+    slot = rng.integers(-2, 40, (50, 12))
+    vm = np.asarray(PP.DEFAULT_VMAX)
+    np.testing.assert_array_equal(synth.gather_vmax(slot, vm), gather_vmax(slot, vm))
+
+
+def test_pack_unpack_round_trip():
+    f, slot, _ = synth.make_cells(37, seed=1)
+    a = abi.pack(f, 37, dtype=np.float64, pitch=64)
+    back = abi.unpack(a, 37)
+    for k, v in f.items():
+        np.testing.assert_array_equal(back[k], np.asarray(v).reshape(back[k].shape))
+
+
+def test_synthetic_cells_are_batch_independent():
+    # a cell's draws depend on (seed, cell) only, so any batching gives the same cells
+    f1, s1, l1 = synth.make_cells(100, seed=5)
+    f2, s2, l2 = synth.make_cells(40, seed=5, cell0=60)
+    np.testing.assert_array_equal(s1[60:], s2)
+    for k in f1:
+        np.testing.assert_array_equal(np.asarray(f1[k])[60:], f2[k])

@@ -1,0 +1,103 @@
+"""Parity of the HIP step with the oracle on the GPU (through the C ABI).
+
+Every failing cell must lie in the oracle's own delta band (cells whose oracle result
+moves beyond the tolerance under an input perturbation at the implementation's rounding
+scale, or whose snow-skin implicit weight puts the blended temperature on tmelt); the band
+and the failures are counted and printed.
+"""
+import numpy as np
+import pytest
+import torch
+
+from ibis_amd import layout as L, synth
+from ibis_amd.params import default_sim_params, get_constants
+from ibis_amd.state import IBISState
+from ibis_amd.step import timestep_body, timestep_
+from oracle import compare, params as OP
+
+from _helpers import OUT_KEYS, assert_parity, failing_cells, oracle_with_band, workload
+
+pytestmark = pytest.mark.gpu
+
+DT = {"f64": torch.float64, "mixed": torch.float32, "f32": torch.float32}
+
+
+def _gpu(f, slot, prec, bedrock=True, pitch=None):
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda",
+                              dtype=DT[prec], pitch=pitch, bedrock=bedrock)
+    new = timestep_body(st, get_constants(), default_sim_params(bedrock=bedrock), 3600.0,
+                        precision=prec)
+    torch.cuda.synchronize()
+    return new.to_numpy()
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("hour", [3, 13, 20])
+def test_step_matches_oracle(prec, hour):
+    n = 6000
+    f, slot = workload(n, seed=21 + hour, hour=hour, prec=prec)
+    band, ref, keys, tol = oracle_with_band(f, slot, prec)
+    got = _gpu(f, slot, prec)
+    assert_parity(got, ref, band, keys, prec, f"gpu hour {hour}")
+
+
+def test_step_without_bedrock_matches_oracle():
+    n = 3000
+    f, slot = workload(n, seed=5, hour=10, prec="f64")
+    f.pop("soil.tsoi_deep")
+    band, ref, keys, tol = oracle_with_band(f, slot, "f64", bedrock=False)
+    got = _gpu(f, slot, "f64", bedrock=False)
+    bad = failing_cells(got, ref, keys, tol)
+    assert not (bad & ~band).any()
+
+
+def test_padded_pitch_and_ragged_sizes():
+    for n in (1, 63, 257):
+        f, slot = workload(n, seed=9, hour=12, prec="f64")
+        a = _gpu(f, slot, "f64")
+        b = _gpu(f, slot, "f64", pitch=n + 37)
+        for k in OUT_KEYS:
+            np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_inplace_equals_functional():
+    n = 2048
+    f, slot = workload(n, seed=3, hour=15, prec="mixed")
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda")
+    c, p = get_constants(), default_sim_params()
+    new = timestep_body(st, c, p, 3600.0)
+    st2 = st.clone()
+    timestep_(st2, c, p, 3600.0)
+    torch.cuda.synchronize()
+    assert torch.equal(new.arena, st2.arena)
+    assert new.nytimes == st2.nytimes == 1.0
+
+
+def test_functional_step_leaves_input_untouched():
+    n = 1000
+    f, slot = workload(n, seed=4, hour=9, prec="mixed")
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda")
+    before = st.arena.clone()
+    timestep_body(st, get_constants(), default_sim_params(), 3600.0)
+    torch.cuda.synchronize()
+    assert torch.equal(before, st.arena)
+
+
+def test_day_of_hours_stays_finite_and_bounded():
+    n = 4096
+    f, slot, lat = synth.make_cells(n, seed=8)
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda")
+    c, p = get_constants(), default_sim_params()
+    for h in range(24):
+        forc = synth.forcing(n, 8, h, lat, f["atm.psurf"])
+        for k, v in forc.items():
+            st.field(k).copy_(torch.as_tensor(v, dtype=torch.float32).reshape(st.field(k).shape))
+        timestep_(st, c, p, 3600.0)
+    torch.cuda.synchronize()
+    out = st.to_numpy()
+    for k in ("soil.tsoi", "soil.wsoi", "canopy.tu", "soil.tsoi_deep", "snow.fi"):
+        assert np.isfinite(out[k]).mean() > 0.999, k
+    t = out["soil.tsoi"][np.isfinite(out["soil.tsoi"])]
+    assert t.min() > 200 and t.max() < 360
+    w = out["soil.wsoi"][np.isfinite(out["soil.wsoi"])]
+    assert w.min() >= 0.0 and w.max() <= 1.0

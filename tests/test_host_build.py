@@ -1,0 +1,80 @@
+"""The kernel source, built for the host, against the oracle (no GPU needed).
+
+``libibis_hostcheck.so`` compiles ``ibis_step_cell.hpp`` -- the same per-cell source the
+HIP kernel runs -- with g++.  It checks the kernel's logic on the CPU; the GPU tests
+(test_gpu_step.py) check the compiled kernel itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from ibis_amd import abi, synth
+from ibis_amd.params import SnowConstants, default_sim_params, get_constants
+from oracle import compare, params as OP
+
+from _helpers import assert_parity, failing_cells, oracle_with_band, workload
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ibis_amd",
+                   "libibis_hostcheck.so")
+CODE = {"f64": abi.PREC_F64, "mixed": abi.PREC_MIXED, "f32": abi.PREC_F32}
+
+
+def host_step(f, slot, prec, bedrock=True, nytimes=0.0):
+    lib = C.CDLL(LIB)
+    n = len(f["soil.tg"])
+    consts, params = get_constants(), default_sim_params(bedrock=bedrock)
+    p = abi.params_from(consts, params, SnowConstants(), synth.HSOI, OP.root_fractions(), 380e-6)
+    a = abi.pack(f, n, dtype=np.float64 if prec == "f64" else np.float32)
+    o = a.copy()
+    s = abi.pack_slots(slot, n)
+    tab = synth.type_table(params.stomata_params.vmax_pft)
+    flags = CODE[prec] | (abi.FLAG_BEDROCK if bedrock else 0)
+    rc = lib.ibis_hostcheck_step(a.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p),
+                                 C.c_int64(n), C.c_int64(n), s.ctypes.data_as(C.c_void_p),
+                                 C.c_int64(n), tab.ctypes.data_as(C.c_void_p), C.byref(p),
+                                 C.c_double(3600.0), C.c_double(nytimes), C.c_int(flags))
+    assert rc == 0
+    return abi.unpack(o, n)
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("hour", [2, 13])
+def test_host_build_matches_oracle(prec, hour):
+    n = 3000
+    f, slot = workload(n, seed=31 + hour, hour=hour, prec=prec)
+    band, ref, keys, tol = oracle_with_band(f, slot, prec)
+    got = host_step(f, slot, prec)
+    assert_parity(got, ref, band, keys, prec, f"host hour {hour}")
+
+
+def test_host_build_frozen_and_saturated_cells():
+    n = 2000
+    f, slot = workload(n, seed=41, hour=6, prec="f64", frozen=0.3, saturated=0.3)
+    band, ref, keys, tol = oracle_with_band(f, slot, "f64")
+    got = host_step(f, slot, "f64")
+    bad = failing_cells(got, ref, keys, tol)
+    assert not (bad & ~band).any()
+
+
+def test_host_build_without_bedrock():
+    n = 1000
+    f, slot = workload(n, seed=43, hour=11, prec="f64")
+    f.pop("soil.tsoi_deep")
+    band, ref, keys, tol = oracle_with_band(f, slot, "f64", bedrock=False)
+    got = host_step(f, slot, "f64", bedrock=False)
+    bad = failing_cells(got, ref, keys, tol)
+    assert not (bad & ~band).any()
+
+
+def test_plant_type_slots_change_vmax():
+    # every slot generic (-1) reads the IBIS PFT table; zone types read their parent's value
+    n = 200
+    f, slot = workload(n, seed=44, hour=13, prec="f64")
+    generic = np.full_like(slot, -1)
+    a = host_step(f, slot, "f64")
+    b = host_step(f, generic, "f64")
+    assert not np.allclose(a["canopy.ag_upper"], b["canopy.ag_upper"])
+    band, ref, keys, tol = oracle_with_band(f, generic, "f64")
+    assert_parity(b, ref, band, keys, "f64", "generic slots")
