@@ -43,6 +43,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-hours", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the hours, copy a buffer of this many MiB once (a kernel of "
+                         "known traffic that calibrates the HBM byte counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has a single MI355X; cells "
@@ -95,6 +98,14 @@ def main():
         if ev is not None:
             ev[1].record(stream)
 
+    if args.copy_probe_mib > 0:
+        words = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(words, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)          # elementwise kernel: reads and writes the size once
+        torch.cuda.synchronize()
+        del src, dst
     for h in range(args.warmup):
         hour(h)
     torch.cuda.synchronize()

@@ -201,6 +201,13 @@ def turvap(s, c, dtime, beta_power=1.0):
     tl = wl * s.tl + (1.0 - wl) * s.tlold
     tg = wg * s.tg + (1.0 - wg) * s.tgold
     ti = wi * s.ti + (1.0 - wi) * s.tiold
+    # a fractional snow-skin weight puts the blend on tmelt up to rounding, where esat/desat
+    # switch branch; ``knife_side`` (+1 / -1) forces the liquid / ice side so that a test
+    # can check both outcomes instead of excluding the cell
+    side = getattr(s, "knife_side", None)
+    if side:
+        kf = (wi > 0.0) & (wi < 1.0)
+        ti = np.where(kf, c.tmelt if side > 0 else np.nextafter(c.tmelt, 0.0), ti)
     ps = s.psurf
 
     def sat(t):

@@ -33,7 +33,7 @@ def gather_vmax(slot_type, vmax_base, overrides=None):
 
 
 def run_step(fields, slot_type, consts=None, params=None, dtime=3600.0, hsoi=None, froot=None,
-             co2conc=380.0e-6, nytimes=0.0, snow=None, bedrock=True):
+             co2conc=380.0e-6, nytimes=0.0, snow=None, bedrock=True, knife_sides=None):
     consts = consts or OP.physical_constants()
     params = params or OP.sim_params()
     p = copy.copy(params)
@@ -46,7 +46,7 @@ def run_step(fields, slot_type, consts=None, params=None, dtime=3600.0, hsoi=Non
     glob = SimpleNamespace(hsoi=OP.HSOI_LAYERS if hsoi is None else hsoi,
                            froot=OP.root_fractions() if froot is None else froot,
                            co2conc=co2conc, nytimes=nytimes,
-                           snow=snow or OP.snow_constants())
+                           snow=snow or OP.snow_constants(), knife_sides=knife_sides)
     f = dict(fields)
     if not bedrock:
         f.pop("soil.tsoi_deep", None)

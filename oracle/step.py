@@ -119,7 +119,8 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
     res = stom = None
     # cells whose snow-skin implicit weight is fractional in some pass: the blended ti is then
     # tmelt up to rounding, where esat/desat switch branch (structural knife edge)
-    knife = np.zeros(tg0.shape, dtype=bool)
+    knife = np.zeros(tg0.shape + (3,), dtype=bool)
+    sides = getattr(glob, "knife_sides", None) or (None, None, None)
     chux = cp_.chu + c.ch2o * wliqu + c.cice * wsnou
     chsx = cp_.chs + c.ch2o * wliqs + c.cice * wsnos
     chlx = cp_.chl + c.ch2o * wliql + c.cice * wsnol
@@ -139,7 +140,7 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
         wl = can_.impexp(tl, chlx, wliql, wsnol, c, wprev[2], first)
         wg = can_.impexp(tg, chgx, wlgx, wigx, c, wprev[3], first)
         wi = can_.impexp2(ti, ti0, c, wprev[4], first)
-        knife = knife | ((wi > 0.0) & (wi < 1.0))
+        knife[:, it] = (wi > 0.0) & (wi < 1.0)
         s = SimpleNamespace(
             tu=tu, ts=ts, tl=tl, t12=t12, t34=t34, q12=q12, q34=q34, tg=tg, ti=ti,
             tuold=tu0, tsold=ts0, tlold=tl0, tgold=tg0, tiold=ti0,
@@ -159,7 +160,8 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
             solu=rad.solu, sols=rad.sols, soll=rad.soll, solg=rad.solg, soli=rad.soli,
             firu=rad.firu, firs=rad.firs, firl=rad.firl, firg=rad.firg, firi=rad.firi,
             firb=rad.firb, pfluxu=cas.pfluxu, pfluxs=cas.pfluxs, pfluxl=cas.pfluxl,
-            ta=ta, qa=qa, psurf=psurf, wu=wu, ws=ws, wl=wl, wg=wg, wi=wi)
+            ta=ta, qa=qa, psurf=psurf, wu=wu, ws=ws, wl=wl, wg=wg, wi=wi,
+            knife_side=sides[it])
         res = can_.turvap(s, c, dtime, beta_power=cp_.soil_beta_power)
         tu, ts, tl, t12, t34 = res.tu, res.ts, res.tl, res.t12, res.t34
         q12, q34, tg, ti = res.q12, res.q34, res.tg, res.ti
