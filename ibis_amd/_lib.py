@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libibis_step.so")
 
 #: every entry point declared in include/ibis_step.h
-EXPORTS = ("ibis_step", "ibis_last_error", "ibis_version")
+EXPORTS = ("ibis_step", "ibis_step_workspace_bytes", "ibis_last_error", "ibis_version")
 
 _lib = None
 
@@ -32,9 +32,11 @@ def load():
                           "(hipcc --offload-arch=gfx950) first")
     lib = C.CDLL(LIB_PATH)
     lib.ibis_step.argtypes = [C.POINTER(IbisArena), C.POINTER(IbisArena), C.c_void_p, C.c_int64,
-                              C.c_void_p, C.POINTER(IbisParams), C.c_double, C.c_double,
-                              C.c_int64, C.c_int, C.c_void_p]
+                              C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(IbisParams),
+                              C.c_double, C.c_double, C.c_int64, C.c_int, C.c_void_p]
     lib.ibis_step.restype = C.c_int
+    lib.ibis_step_workspace_bytes.argtypes = [C.c_int64, C.c_int]
+    lib.ibis_step_workspace_bytes.restype = C.c_int64
     lib.ibis_last_error.restype = C.c_char_p
     lib.ibis_version.restype = C.c_char_p
     _lib = lib

@@ -6,6 +6,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "ibis_step_cell.hpp"
 
 namespace {
@@ -14,13 +16,11 @@ template <class S, class R, class D>
 void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
          int64_t slot_pitch, const double* table, const IbisParams& P, const ibis::StepConst& K,
          int copy_in) {
+  // the three phases over all cells, in the GPU's kernel order
+  std::vector<D> wd((size_t)ibis::WD_N * (size_t)pitch);
+  std::vector<R> wr((size_t)ibis::WR_N * (size_t)pitch);
+  auto work = [&](int64_t i) { return ibis::Work<R, D>{wd.data(), wr.data(), pitch, i}; };
   for (int64_t i = 0; i < ncell; ++i) {
-    R vrow[IBIS_NPFT];
-    for (int j = 0; j < IBIS_NPFT; ++j) {
-      const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
-      const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
-      vrow[j] = R(table[row]);
-    }
     if (copy_in) {
 #define X(F, W, OFF, ROLE)                                                            \
   if ((ROLE) == IBIS_ROLE_IN)                                                         \
@@ -29,8 +29,22 @@ void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
       IBIS_FIELD_TABLE
 #undef X
     }
-    ibis::Cell<S> c{in, out, pitch, i};
-    ibis::step_cell<S, R, D>(c, vrow, P, K);
+    const ibis::Cell<S> c{in, out, pitch, i};
+    ibis::surface_cell<S, R, D>(c, work(i), P, K);
+  }
+  for (int64_t i = 0; i < ncell; ++i) {
+    R vrow[IBIS_NPFT];
+    for (int j = 0; j < IBIS_NPFT; ++j) {
+      const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
+      const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
+      vrow[j] = R(table[row]);
+    }
+    const ibis::Cell<S> c{in, out, pitch, i};
+    ibis::canopy_cell<S, R, D>(c, work(i), vrow, P, K);
+  }
+  for (int64_t i = 0; i < ncell; ++i) {
+    const ibis::Cell<S> c{in, out, pitch, i};
+    ibis::ground_cell<S, R, D>(c, work(i), P, K);
   }
 }
 

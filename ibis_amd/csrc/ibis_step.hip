@@ -1,8 +1,11 @@
 // HIP kernels and C ABI of the hourly step (include/ibis_step.h).
 //
-// One thread per land cell, 256 threads (four wave64) per block.  The plant-type table
-// (52 rows) is staged once per block into LDS and each thread gathers its 12 slot values
-// from it; every state plane is read and written with unit stride across the wavefront.
+// One thread per land cell.  The step runs as three kernels in stream order -- surface,
+// canopy, ground (ibis_step_cell.hpp) -- so that each is register-allocated on its own; the
+// values they hand on live in a plane-major workspace.  Every state and workspace plane is
+// read and written with unit stride across the wavefront.  The canopy kernel stages the
+// plant-type table (52 rows) in LDS once per block and gathers each cell's 12 slot values
+// from it.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -15,11 +18,38 @@ namespace ibis {
 
 constexpr int kBlock = 256;
 
+template <class R, class D>
+__device__ IBIS_INLINE Work<R, D> work_at(void* ws, int64_t pitch, int64_t i) {
+  D* d = (D*)ws;
+  R* r = (R*)(d + (int64_t)WD_N * pitch);
+  return Work<R, D>{d, r, pitch, i};
+}
+
 template <class S, class R, class D>
-__global__ __launch_bounds__(kBlock) void step_kernel(const S* in, S* out, int64_t pitch, int64_t ncell,
-                                                      const int8_t* slot, int64_t slot_pitch,
-                                                      const double* table, IbisParams P,
-                                                      StepConst K, int copy_in) {
+__global__ __launch_bounds__(kBlock) void surface_kernel(const S* in, S* out, void* ws,
+                                                         int64_t pitch, int64_t ncell,
+                                                         IbisParams P, StepConst K, int copy_in) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  if (copy_in) {
+#define X(F, W, OFF, ROLE)                                                      \
+  if ((ROLE) == IBIS_ROLE_IN) {                                                 \
+    for (int j = 0; j < (W); ++j)                                               \
+      out[(int64_t)((OFF) + j) * pitch + i] = in[(int64_t)((OFF) + j) * pitch + i]; \
+  }
+    IBIS_FIELD_TABLE
+#undef X
+  }
+  const Cell<S> c{in, out, pitch, i};
+  surface_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock) void canopy_kernel(const S* in, S* out, void* ws,
+                                                        int64_t pitch, int64_t ncell,
+                                                        const int8_t* slot, int64_t slot_pitch,
+                                                        const double* table, IbisParams P,
+                                                        StepConst K) {
   __shared__ R trow[IBIS_TYPE_ROWS];
   for (int t = threadIdx.x; t < IBIS_TYPE_ROWS; t += blockDim.x) trow[t] = R(table[t]);
   __syncthreads();
@@ -32,17 +62,18 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const S* in, S* out, int64
     const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
     vrow[j] = trow[row];
   }
-  if (copy_in) {
-#define X(F, W, OFF, ROLE)                                                      \
-  if ((ROLE) == IBIS_ROLE_IN) {                                                 \
-    for (int j = 0; j < (W); ++j)                                               \
-      out[(int64_t)((OFF) + j) * pitch + i] = in[(int64_t)((OFF) + j) * pitch + i]; \
-  }
-    IBIS_FIELD_TABLE
-#undef X
-  }
-  Cell<S> c{in, out, pitch, i};
-  step_cell<S, R, D>(c, vrow, P, K);
+  const Cell<S> c{in, out, pitch, i};
+  canopy_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), vrow, P, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock) void ground_kernel(const S* in, S* out, void* ws,
+                                                        int64_t pitch, int64_t ncell,
+                                                        IbisParams P, StepConst K) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  const Cell<S> c{in, out, pitch, i};
+  ground_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
 }
 
 static thread_local char g_err[512] = "";
@@ -53,13 +84,19 @@ static int fail(int code, const char* msg) {
 }
 
 template <class S, class R, class D>
-static int launch(const IbisArena* in, const IbisArena* out, const int8_t* slot,
+static int launch(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
                   int64_t slot_pitch, const double* table, const IbisParams& P,
                   const StepConst& K, int64_t ncell, int copy_in, hipStream_t st) {
-  const int64_t nblk = (ncell + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((step_kernel<S, R, D>), dim3((unsigned)nblk), dim3(kBlock), 0, st,
-                     (const S*)in->planes, (S*)out->planes, in->pitch,
-                     ncell, slot, slot_pitch, table, P, K, copy_in);
+  const dim3 grid((unsigned)((ncell + kBlock - 1) / kBlock)), block(kBlock);
+  const S* pin = (const S*)in->planes;
+  S* pout = (S*)out->planes;
+  const int64_t pitch = in->pitch;
+  hipLaunchKernelGGL((surface_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     P, K, copy_in);
+  hipLaunchKernelGGL((canopy_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     slot, slot_pitch, table, P, K);
+  hipLaunchKernelGGL((ground_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     P, K);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     snprintf(g_err, sizeof(g_err), "ibis_step: launch failed: %s", hipGetErrorString(e));
@@ -68,17 +105,32 @@ static int launch(const IbisArena* in, const IbisArena* out, const int8_t* slot,
   return IBIS_OK;
 }
 
+static int64_t workspace_bytes(int64_t pitch, int prec) {
+  switch (prec) {
+    case IBIS_PREC_MIXED: return work_bytes(pitch, (int)sizeof(float), (int)sizeof(double));
+    case IBIS_PREC_F32: return work_bytes(pitch, (int)sizeof(float), (int)sizeof(float));
+    default: return work_bytes(pitch, (int)sizeof(double), (int)sizeof(double));
+  }
+}
+
 }  // namespace ibis
 
 extern "C" {
 
 const char* ibis_last_error(void) { return ibis::g_err; }
 
-const char* ibis_version(void) { return "ibis_amd 0.1 (gfx950)"; }
+const char* ibis_version(void) { return "ibis_amd 0.2 (gfx950, three-phase step)"; }
 
-int ibis_step(const IbisArena* in, const IbisArena* out, const int8_t* slot_type,
-              int64_t slot_pitch, const double* type_table, const IbisParams* params,
-              double dtime, double nytimes, int64_t ncell, int flags, void* hip_stream) {
+int64_t ibis_step_workspace_bytes(int64_t pitch, int flags) {
+  const int prec = flags & IBIS_PREC_MASK;
+  if (pitch < 0 || prec > 2) return IBIS_E_ARG;
+  return ibis::workspace_bytes(pitch, prec);
+}
+
+int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
+              int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
+              const double* type_table, const IbisParams* params, double dtime, double nytimes,
+              int64_t ncell, int flags, void* hip_stream) {
   using namespace ibis;
   g_err[0] = 0;
   if (!in || !out || !params || !type_table || !in->planes || !out->planes)
@@ -92,6 +144,11 @@ int ibis_step(const IbisArena* in, const IbisArena* out, const int8_t* slot_type
   const int prec = flags & IBIS_PREC_MASK;
   if ((flags & ~(IBIS_PREC_MASK | IBIS_FLAG_BEDROCK | IBIS_FLAG_COPY_IN)) != 0 || prec > 2)
     return fail(IBIS_E_FLAGS, "ibis_step: unknown flag bits or precision");
+  if (!workspace || workspace_bytes < ibis::workspace_bytes(in->pitch, prec))
+    return fail(IBIS_E_ARG, "ibis_step: workspace missing or smaller than "
+                            "ibis_step_workspace_bytes(pitch, flags)");
+  if (((uintptr_t)workspace & 255u) != 0)
+    return fail(IBIS_E_ARG, "ibis_step: workspace must be 256-byte aligned");
   const IbisParams& P = *params;
   if (flags & IBIS_FLAG_BEDROCK) {
     for (int k = 0; k < IBIS_NBEDROCK; ++k)
@@ -109,14 +166,14 @@ int ibis_step(const IbisArena* in, const IbisArena* out, const int8_t* slot_type
   hipStream_t st = (hipStream_t)hip_stream;
   switch (prec) {
     case IBIS_PREC_MIXED:
-      return launch<float, float, double>(in, out, slot_type, slot_pitch, type_table, P, K, ncell,
-                                          copy_in, st);
+      return launch<float, float, double>(in, out, workspace, slot_type, slot_pitch, type_table,
+                                          P, K, ncell, copy_in, st);
     case IBIS_PREC_F32:
-      return launch<float, float, float>(in, out, slot_type, slot_pitch, type_table, P, K, ncell,
-                                         copy_in, st);
+      return launch<float, float, float>(in, out, workspace, slot_type, slot_pitch, type_table,
+                                         P, K, ncell, copy_in, st);
     default:
-      return launch<double, double, double>(in, out, slot_type, slot_pitch, type_table, P, K,
-                                            ncell, copy_in, st);
+      return launch<double, double, double>(in, out, workspace, slot_type, slot_pitch,
+                                            type_table, P, K, ncell, copy_in, st);
   }
 }
 

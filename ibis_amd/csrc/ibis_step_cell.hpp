@@ -1,5 +1,18 @@
 // The hourly step for one cell: timestep_body (python-port/ibis/timestepping/loops.py:234-1284)
 // over the plane-major arena.  S = storage type, R = bulk arithmetic, D = implicit solves.
+//
+// The step runs as three phases, each a kernel of its own so that each is register-allocated
+// on its own:
+//   surface_cell  setsoi, wet fractions, radiation, interception cascade, aerodynamics,
+//                 dry stress                               (loops.py:285-470)
+//   canopy_cell   the three turcof/stomata/turvap passes    (loops.py:473-611)
+//   ground_cell   cascad2, noveg, flux clamps, snow, soil, fire, temperature bound
+//                                                          (loops.py:616-1275)
+// Values handed from one phase to the next that are not state fields go through a plane-major
+// workspace (WorkD planes in the solve precision D, WorkR planes in the bulk precision R), so
+// each value crosses in the precision it was computed in and the phases reproduce the
+// single-pass arithmetic exactly.  Every phase reads the input arena's io fields before the
+// phase that overwrites them runs, so in-place stepping (out == in) is safe.
 #pragma once
 
 #include "ibis_physics_b.hpp"
@@ -19,6 +32,8 @@ template <class S> struct Cell {
   int64_t i;
   IBIS_HD IBIS_INLINE S ld(int plane) const { return in[(int64_t)plane * pitch + i]; }
   IBIS_HD IBIS_INLINE void st(int plane, S v) const { out[(int64_t)plane * pitch + i] = v; }
+  // a field an earlier phase of this step has written
+  IBIS_HD IBIS_INLINE S lo(int plane) const { return out[(int64_t)plane * pitch + i]; }
 };
 
 // Per-cell trace points for host-side debugging builds; empty in every product build.
@@ -26,17 +41,71 @@ template <class S> struct Cell {
 #define IBIS_TRACE(cell, tag, value) ((void)0)
 #endif
 
+// Workspace planes (per cell).  D planes: canini's aerodynamic block, the pre-solve infrared
+// terms and the transpiration fluxes; R planes: PAR scaling terms, the cascade's stores and
+// fluxes, the final canopy iterate and the flux totals that cascad2 updates.
+enum WorkDPlane {
+  WD_AERO = 0,                 // 26: Aero<D> fields in declaration order
+  WD_IR = WD_AERO + 26,        // 6: firu firs firl firg firi firb
+  WD_FVAPUT = WD_IR + 6,
+  WD_FVAPLT,
+  WD_N
+};
+enum WorkRPlane {
+  WR_TERM = 0,                 // 4: termu6 termu7 terml6 terml7
+  WR_W = WR_TERM + 4,          // 6: wliqu wliqs wliql wsnou wsnos wsnol after the cascade
+  WR_PFLUX = WR_W + 6,         // 3: pfluxu pfluxs pfluxl
+  WR_TRAING = WR_PFLUX + 3,
+  WR_TSNOWG,
+  WR_CUR,                      // 5: tu ts tl tg ti after the last pass (ti clamped)
+  WR_FVAPUW = WR_CUR + 5,
+  WR_FVAPS,
+  WR_FVAPLW,
+  WR_FVAPA,
+  WR_FSENA,
+  WR_N
+};
+
+template <class R, class D> struct Work {
+  D* d;
+  R* r;
+  int64_t pitch;
+  int64_t i;
+  IBIS_HD IBIS_INLINE D ldd(int p) const { return d[(int64_t)p * pitch + i]; }
+  IBIS_HD IBIS_INLINE void std_(int p, D v) const { d[(int64_t)p * pitch + i] = v; }
+  IBIS_HD IBIS_INLINE R ldr(int p) const { return r[(int64_t)p * pitch + i]; }
+  IBIS_HD IBIS_INLINE void str(int p, R v) const { r[(int64_t)p * pitch + i] = v; }
+};
+
+// bytes of workspace for `pitch` cells with bulk / solve element sizes rsize / dsize
+IBIS_HD inline int64_t work_bytes(int64_t pitch, int rsize, int dsize) {
+  return pitch * ((int64_t)WD_N * dsize + (int64_t)WR_N * rsize);
+}
+
+template <class R> IBIS_HD IBIS_INLINE void aero_store(const Aero<R>& a, R* v) {
+  const R t[26] = {a.rhoa, a.cp, a.za, a.z1, a.z2, a.z3, a.z4, a.z12, a.z34, a.bdl, a.bdu,
+                   a.dil, a.diu, a.exphl, a.expl, a.exphu, a.expu, a.alogav, a.alog1, a.alog2,
+                   a.alog3, a.alog4, a.aloga, a.alogu, a.alogl, a.u2};
+  for (int j = 0; j < 26; ++j) v[j] = t[j];
+}
+template <class R> IBIS_HD IBIS_INLINE void aero_load(Aero<R>& a, const R* v) {
+  a.rhoa = v[0]; a.cp = v[1]; a.za = v[2]; a.z1 = v[3]; a.z2 = v[4]; a.z3 = v[5]; a.z4 = v[6];
+  a.z12 = v[7]; a.z34 = v[8]; a.bdl = v[9]; a.bdu = v[10]; a.dil = v[11]; a.diu = v[12];
+  a.exphl = v[13]; a.expl = v[14]; a.exphu = v[15]; a.expu = v[16]; a.alogav = v[17];
+  a.alog1 = v[18]; a.alog2 = v[19]; a.alog3 = v[20]; a.alog4 = v[21]; a.aloga = v[22];
+  a.alogu = v[23]; a.alogl = v[24]; a.u2 = v[25];
+}
+
 #define LD(P) R(c.ld(P))
 #define ST(P, v) c.st((P), S(v))
+#define LO(P) R(c.lo(P))
 
 template <class S, class R, class D>
-IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisParams& P,
-                       const StepConst& K) {
+IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisParams& P,
+                          const StepConst& K) {
   IBIS_TRACE(c.i, "cell", 0);
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
   const R dtime = R(K.dtime);
-  const D dtD = D(K.dtime), epsD = D(P.epsilon), tmeltD = D(P.tmelt);
-  const D ch2oD = D(P.ch2o), ciceD = D(P.cice), rhowD = D(P.rhow), hfusD = D(P.hfus);
   constexpr int NS = IBIS_NSOILAY;
 
   // ---- inputs --------------------------------------------------------------------------
@@ -188,23 +257,118 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
     ST(IBIS_P_SOIL_STRESSTU, stresstu);
   }
 
-  // solid phase (loops.py:190-215) and the intermediates of loops.py:147-183
-  R csoi[NS], rhosoi[NS];
+  // hand-offs to the canopy and ground phases
+  {
+    D av[26];
+    aero_store(aero, av);
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    const R cs_ = LD(IBIS_P_SOIL_CSOI + k), rh_ = LD(IBIS_P_SOIL_RHOSOI + k);
-    csoi[k] = cs_ > R(0) ? cs_ : R(P.csoi_mineral_val);
-    rhosoi[k] = rh_ > R(0) ? rh_ : R(P.rhosoi_val);
+    for (int j = 0; j < 26; ++j) wk.std_(WD_AERO + j, av[j]);
   }
-  const R hsoi0 = R(P.hsoi[0]);
+  wk.std_(WD_IR + 0, ir.firu);
+  wk.std_(WD_IR + 1, ir.firs);
+  wk.std_(WD_IR + 2, ir.firl);
+  wk.std_(WD_IR + 3, ir.firg);
+  wk.std_(WD_IR + 4, ir.firi);
+  wk.std_(WD_IR + 5, ir.firb);
+  wk.str(WR_TERM + 0, rad.termu6);
+  wk.str(WR_TERM + 1, rad.termu7);
+  wk.str(WR_TERM + 2, rad.terml6);
+  wk.str(WR_TERM + 3, rad.terml7);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) wk.str(WR_W + j, cas.w[j]);
+  wk.str(WR_PFLUX + 0, cas.pfluxu);
+  wk.str(WR_PFLUX + 1, cas.pfluxs);
+  wk.str(WR_PFLUX + 2, cas.pfluxl);
+  wk.str(WR_TRAING, cas.traing);
+  wk.str(WR_TSNOWG, cas.tsnowg);
+  ST(IBIS_P_CANOPY_RAING, cas.raing);
+  ST(IBIS_P_CANOPY_SNOWG, cas.snowg);
+  ST(IBIS_P_HYDROLOGY_RAINL, cas.rainl);
+  ST(IBIS_P_HYDROLOGY_SNOWL, cas.snowl);
+  ST(IBIS_P_HYDROLOGY_TRAINL, cas.trainl);
+  ST(IBIS_P_HYDROLOGY_TSNOWL, cas.tsnowl);
+}
+
+template <class S, class R, class D>
+IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
+                         const IbisParams& P, const StepConst& K) {
+  const R eps = R(P.epsilon);
+  const D dtD = D(K.dtime), tmeltD = D(P.tmelt);
+
+  // ---- inputs and the surface phase's results ---------------------------------------------
+  const R tsoi0 = LD(IBIS_P_SOIL_TSOI), wsoi0 = LD(IBIS_P_SOIL_WSOI);
+  const R wisoi0 = LD(IBIS_P_SOIL_WISOI), poros0 = LD(IBIS_P_SOIL_POROS);
+  const R tg0 = LD(IBIS_P_SOIL_TG), wpud = LD(IBIS_P_SOIL_WPUD), wipud = LD(IBIS_P_SOIL_WIPUD);
+  const R fi0 = LD(IBIS_P_SNOW_FI), ti0 = LD(IBIS_P_SNOW_TI);
+  const R tsno0 = LD(IBIS_P_SNOW_TSNO), hsno0 = LD(IBIS_P_SNOW_HSNO);
+  const R tu0 = LD(IBIS_P_CANOPY_TU), ts0 = LD(IBIS_P_CANOPY_TS), tl0 = LD(IBIS_P_CANOPY_TL);
+  const R t12_0 = LD(IBIS_P_CANOPY_T12), t34_0 = LD(IBIS_P_CANOPY_T34);
+  const R ta = LD(IBIS_P_ATM_TA), qa = LD(IBIS_P_ATM_QA), ua = LD(IBIS_P_ATM_UA);
+  const R psurf = LD(IBIS_P_ATM_PSURF);
+  const R lai[2] = {LD(IBIS_P_VEGETATION_LAI), LD(IBIS_P_VEGETATION_LAI + 1)};
+  const R sai[2] = {LD(IBIS_P_VEGETATION_SAI), LD(IBIS_P_VEGETATION_SAI + 1)};
+  const R fu = LD(IBIS_P_VEGETATION_FU), fl = LD(IBIS_P_VEGETATION_FL);
+  const R cs0 = LD(IBIS_P_SOIL_CSOI), rh0 = LD(IBIS_P_SOIL_RHOSOI);
+  const R csoi0 = cs0 > R(0) ? cs0 : R(P.csoi_mineral_val);
+  const R rhosoi0 = rh0 > R(0) ? rh0 : R(P.rhosoi_val);
+  Aero<D> aero;
+  {
+    D av[26];
+#pragma unroll
+    for (int j = 0; j < 26; ++j) av[j] = wk.ldd(WD_AERO + j);
+    aero_load(aero, av);
+  }
+  RadOut<R> rad;
+  rad.solu = LO(IBIS_P_CANOPY_SOLU);
+  rad.sols = LO(IBIS_P_CANOPY_SOLS);
+  rad.soll = LO(IBIS_P_CANOPY_SOLL);
+  rad.solg = LO(IBIS_P_CANOPY_SOLG);
+  rad.soli = LO(IBIS_P_CANOPY_SOLI);
+  rad.topparu = LO(IBIS_P_CANOPY_TOPPARU);
+  rad.topparl = LO(IBIS_P_CANOPY_TOPPARL);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    rad.scalcoefu[j] = LO(IBIS_P_CANOPY_SCALCOEFU + j);
+    rad.scalcoefl[j] = LO(IBIS_P_CANOPY_SCALCOEFL + j);
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) rad.asurd[b] = rad.asuri[b] = R(0);
+  rad.termu6 = wk.ldr(WR_TERM + 0);
+  rad.termu7 = wk.ldr(WR_TERM + 1);
+  rad.terml6 = wk.ldr(WR_TERM + 2);
+  rad.terml7 = wk.ldr(WR_TERM + 3);
+  IrOut<D> ir;
+  ir.firu = wk.ldd(WD_IR + 0);
+  ir.firs = wk.ldd(WD_IR + 1);
+  ir.firl = wk.ldd(WD_IR + 2);
+  ir.firg = wk.ldd(WD_IR + 3);
+  ir.firi = wk.ldd(WD_IR + 4);
+  ir.firb = wk.ldd(WD_IR + 5);
+  Fwet<R> fw2;
+  fw2.fwetu = LO(IBIS_P_CANOPY_FWETU);
+  fw2.fwets = LO(IBIS_P_CANOPY_FWETS);
+  fw2.fwetl = LO(IBIS_P_CANOPY_FWETL);
+  fw2.rliqu = LO(IBIS_P_CANOPY_RLIQU);
+  fw2.rliqs = LO(IBIS_P_CANOPY_RLIQS);
+  fw2.rliql = LO(IBIS_P_CANOPY_RLIQL);
+  const R consoi0 = LO(IBIS_P_SOIL_CONSOI);
+  const R hvasug = LO(IBIS_P_SOIL_HVASUG), hvasui = LO(IBIS_P_SNOW_HVASUI);
+  const R stresstu = LO(IBIS_P_SOIL_STRESSTU), stresstl = LO(IBIS_P_SOIL_STRESSTL);
+  R casw[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) casw[j] = wk.ldr(WR_W + j);
+  const R pfluxu = wk.ldr(WR_PFLUX + 0), pfluxs = wk.ldr(WR_PFLUX + 1);
+  const R pfluxl = wk.ldr(WR_PFLUX + 2);
   const R xu = R(2) * lai[1] * fu;
   const R xs = R(2) * sai[1] * fu;
   const R xl = R(2) * (lai[0] + sai[0]) * fl * (R(1) - fi0);
 
+  const R hsoi0 = R(P.hsoi[0]);
+
   TvFix<D> F;
   {
-    const R cog = consoi[0] / mx(R(0.5) * hsoi0, eps);
-    const R coi = R(P.consno) / mx(R(0.5) * mx(hsno[0], R(P.hsnotop)), eps);
+    const R cog = consoi0 / mx(R(0.5) * hsoi0, eps);
+    const R coi = R(P.consno) / mx(R(0.5) * mx(hsno0, R(P.hsnotop)), eps);
     const R zirg = R(4.0 * 0.95) * R(P.stef) * cube(tg0);
     const R ziri = R(4.0 * 0.95) * R(P.stef) * cube(ti0);
     F.tuold = tu0; F.tsold = ts0; F.tlold = tl0; F.tgold = tg0; F.tiold = ti0;
@@ -212,31 +376,31 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
     F.fu = fu; F.fl = fl; F.lai0 = lai[0]; F.lai1 = lai[1]; F.sai0 = sai[0]; F.sai1 = sai[1];
 #pragma unroll
     for (int j = 0; j < IBIS_NPFT; ++j) F.frac[j] = D(LD(IBIS_P_VEGETATION_FRAC + j));
-    F.chux = R(P.chu) + R(P.ch2o) * cas.w[0] + R(P.cice) * cas.w[3];
-    F.chsx = R(P.chs) + R(P.ch2o) * cas.w[1] + R(P.cice) * cas.w[4];
-    F.chlx = R(P.chl) + R(P.ch2o) * cas.w[2] + R(P.cice) * cas.w[5];
+    F.chux = R(P.chu) + R(P.ch2o) * casw[0] + R(P.cice) * casw[3];
+    F.chsx = R(P.chs) + R(P.ch2o) * casw[1] + R(P.cice) * casw[4];
+    F.chlx = R(P.chl) + R(P.ch2o) * casw[2] + R(P.cice) * casw[5];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) F.w[j] = cas.w[j];
+    for (int j = 0; j < 6; ++j) F.w[j] = casw[j];
     F.rliqu = fw2.rliqu; F.rliqs = fw2.rliqs; F.rliql = fw2.rliql;
     F.fwetu = fw2.fwetu; F.fwets = fw2.fwets; F.fwetl = fw2.fwetl;
     F.cog = cog; F.coi = coi; F.zirg = zirg; F.ziri = ziri;
-    F.wpud = wpud; F.wipud = wipud; F.wisoi1 = wisoi[0]; F.wsoi1 = wsoi[0];
+    F.wpud = wpud; F.wipud = wipud; F.wisoi1 = wisoi0; F.wsoi1 = wsoi0;
     F.swilt1 = LD(IBIS_P_SOIL_SWILT); F.suction1 = LD(IBIS_P_SOIL_SUCTION);
     F.bex1 = LD(IBIS_P_SOIL_BEX); F.hvasug = hvasug; F.hvasui = hvasui;
-    F.tsoi1 = tsoi[0]; F.tsno1 = tsno[0];
+    F.tsoi1 = tsoi0; F.tsno1 = tsno0;
     F.solu = rad.solu; F.sols = rad.sols; F.soll = rad.soll; F.solg = rad.solg;
     F.soli = rad.soli; F.firu = ir.firu; F.firs = ir.firs; F.firl = ir.firl;
     F.firg = ir.firg; F.firi = ir.firi; F.firb = ir.firb;
-    F.pfluxu = cas.pfluxu; F.pfluxs = cas.pfluxs; F.pfluxl = cas.pfluxl;
+    F.pfluxu = pfluxu; F.pfluxs = pfluxs; F.pfluxl = pfluxl;
     F.ta = ta; F.qa = qa; F.psurf = psurf;
   }
   // ground heat capacity and water for the ground implicit weight (loops.py:164-181)
-  const R rwp = poros[0] * R(P.rhow);
+  const R rwp = poros0 * R(P.rhow);
   const R chgx = R(P.ch2o) * wpud + R(P.cice) * wipud +
-                 ((R(1) - poros[0]) * csoi[0] * rhosoi[0] +
-                  rwp * (R(1) - wisoi[0]) * wsoi[0] * R(P.ch2o) + rwp * wisoi[0] * R(P.cice)) * hsoi0;
-  const R wlgx = wpud + rwp * (R(1) - wisoi[0]) * wsoi[0] * hsoi0;
-  const R wigx = wipud + rwp * wisoi[0] * hsoi0;
+                 ((R(1) - poros0) * csoi0 * rhosoi0 +
+                  rwp * (R(1) - wisoi0) * wsoi0 * R(P.ch2o) + rwp * wisoi0 * R(P.cice)) * hsoi0;
+  const R wlgx = wpud + rwp * (R(1) - wisoi0) * wsoi0 * hsoi0;
+  const R wigx = wipud + rwp * wisoi0 * hsoi0;
 
   // Vmax of the big leaves by existence priority (stomata.py:598-650)
   R ex[5];
@@ -336,28 +500,87 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
   ST(IBIS_P_CANOPY_FIRG, tv.firg);
   ST(IBIS_P_CANOPY_FIRI, tv.firi);
   ST(IBIS_P_CANOPY_FIRB, tv.firb);
-  ST(IBIS_P_CANOPY_RAING, cas.raing);
-  ST(IBIS_P_CANOPY_SNOWG, cas.snowg);
   ST(IBIS_P_HYDROLOGY_GINVAP, tv.ginvap);
   ST(IBIS_P_HYDROLOGY_GSUVAP, tv.gsuvap);
   ST(IBIS_P_HYDROLOGY_GTRANS, tv.gtrans);
-  ST(IBIS_P_HYDROLOGY_RAINL, cas.rainl);
-  ST(IBIS_P_HYDROLOGY_SNOWL, cas.snowl);
-  ST(IBIS_P_HYDROLOGY_TRAINL, cas.trainl);
-  ST(IBIS_P_HYDROLOGY_TSNOWL, cas.tsnowl);
   ST(IBIS_P_HYDROLOGY_WTOT, R(0));
 
+  // hand-offs to the ground phase
+  wk.str(WR_CUR + 0, R(cur[0]));
+  wk.str(WR_CUR + 1, R(cur[1]));
+  wk.str(WR_CUR + 2, R(cur[2]));
+  wk.str(WR_CUR + 3, tg);
+  wk.str(WR_CUR + 4, ti);
+  wk.str(WR_FVAPUW, R(tv.fvapuw));
+  wk.str(WR_FVAPS, R(tv.fvaps));
+  wk.str(WR_FVAPLW, R(tv.fvaplw));
+  wk.str(WR_FVAPA, R(tv.fvapa));
+  wk.str(WR_FSENA, R(tv.fsena));
+  wk.std_(WD_FVAPUT, tv.fvaput);
+  wk.std_(WD_FVAPLT, tv.fvaplt);
+}
+
+template <class S, class R, class D>
+IBIS_HD void ground_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisParams& P,
+                         const StepConst& K) {
+  const R eps = R(P.epsilon), tmelt = R(P.tmelt);
+  const R dtime = R(K.dtime);
+  const D dtD = D(K.dtime), epsD = D(P.epsilon), tmeltD = D(P.tmelt);
+  const D ch2oD = D(P.ch2o), ciceD = D(P.cice), rhowD = D(P.rhow), hfusD = D(P.hfus);
+  constexpr int NS = IBIS_NSOILAY;
+
+  // ---- inputs and the earlier phases' results -----------------------------------------------
+  R tsoi[NS], wsoi[NS], wisoi[NS], poros[NS], csoi[NS], rhosoi[NS], consoi[NS];
+  R stressl[NS], stressu[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    tsoi[k] = LD(IBIS_P_SOIL_TSOI + k);
+    wsoi[k] = LD(IBIS_P_SOIL_WSOI + k);
+    wisoi[k] = LD(IBIS_P_SOIL_WISOI + k);
+    poros[k] = LD(IBIS_P_SOIL_POROS + k);
+    const R cs_ = LD(IBIS_P_SOIL_CSOI + k), rh_ = LD(IBIS_P_SOIL_RHOSOI + k);
+    csoi[k] = cs_ > R(0) ? cs_ : R(P.csoi_mineral_val);
+    rhosoi[k] = rh_ > R(0) ? rh_ : R(P.rhosoi_val);
+    consoi[k] = LO(IBIS_P_SOIL_CONSOI + k);
+    stressl[k] = LO(IBIS_P_SOIL_STRESSL + k);
+    stressu[k] = LO(IBIS_P_SOIL_STRESSU + k);
+  }
+  const R stresstl = LO(IBIS_P_SOIL_STRESSTL), stresstu = LO(IBIS_P_SOIL_STRESSTU);
+  const R q1 = LO(IBIS_P_SOIL_QGLIF + 0), q2 = LO(IBIS_P_SOIL_QGLIF + 1);
+  const R q3 = LO(IBIS_P_SOIL_QGLIF + 2), q4 = LO(IBIS_P_SOIL_QGLIF + 3);
+  const R hvasug = LO(IBIS_P_SOIL_HVASUG), hvasui = LO(IBIS_P_SNOW_HVASUI);
+  const R wpud = LD(IBIS_P_SOIL_WPUD), wipud = LD(IBIS_P_SOIL_WIPUD);
+  const R fi0 = LD(IBIS_P_SNOW_FI), tlsub = LD(IBIS_P_SNOW_TLSUB);
+  R tsno[3], hsno[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    tsno[k] = LD(IBIS_P_SNOW_TSNO + k);
+    hsno[k] = LD(IBIS_P_SNOW_HSNO + k);
+  }
+  const R ta = LD(IBIS_P_ATM_TA), qa = LD(IBIS_P_ATM_QA), psurf = LD(IBIS_P_ATM_PSURF);
+  const R lai[2] = {LD(IBIS_P_VEGETATION_LAI), LD(IBIS_P_VEGETATION_LAI + 1)};
+  const R sai[2] = {LD(IBIS_P_VEGETATION_SAI), LD(IBIS_P_VEGETATION_SAI + 1)};
+  const R fu = LD(IBIS_P_VEGETATION_FU), fl = LD(IBIS_P_VEGETATION_FL);
+  const R ztop[2] = {LD(IBIS_P_VEGETATION_ZTOP), LD(IBIS_P_VEGETATION_ZTOP + 1)};
+  const R raing = LO(IBIS_P_CANOPY_RAING), snowg = LO(IBIS_P_CANOPY_SNOWG);
+  const R rainl = LO(IBIS_P_HYDROLOGY_RAINL), snowl = LO(IBIS_P_HYDROLOGY_SNOWL);
+  const R trainl = LO(IBIS_P_HYDROLOGY_TRAINL), tsnowl = LO(IBIS_P_HYDROLOGY_TSNOWL);
+  const R traing = wk.ldr(WR_TRAING), tsnowg = wk.ldr(WR_TSNOWG);
+  const R fvapg = LO(IBIS_P_CANOPY_FVAPG), fvapi = LO(IBIS_P_CANOPY_FVAPI);
+  R tg = wk.ldr(WR_CUR + 3);
+  const R ti = wk.ldr(WR_CUR + 4);
+
   // ---- 7. cascad2 (interception.py:423-463) -------------------------------------------------
-  R tu = R(cur[0]), ts = R(cur[1]), tl = R(cur[2]);
+  R tu = wk.ldr(WR_CUR + 0), ts = wk.ldr(WR_CUR + 1), tl = wk.ldr(WR_CUR + 2);
   R w[6];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) w[j] = cas.w[j];
-  R fvapa = R(tv.fvapa), fsena = R(tv.fsena);
-  steph2o2(tu, w[0], w[3], fu, R(2) * lai[1], fw2.rliqu, R(tv.fvapuw), R(P.chu), dtime, ta, fvapa,
+  for (int j = 0; j < 6; ++j) w[j] = wk.ldr(WR_W + j);
+  R fvapa = wk.ldr(WR_FVAPA), fsena = wk.ldr(WR_FSENA);
+  steph2o2(tu, w[0], w[3], fu, R(2) * lai[1], LO(IBIS_P_CANOPY_RLIQU), wk.ldr(WR_FVAPUW), R(P.chu), dtime, ta, fvapa,
            fsena, P);
-  steph2o2(ts, w[1], w[4], fu, R(2) * sai[1], fw2.rliqs, R(tv.fvaps), R(P.chs), dtime, ta, fvapa,
+  steph2o2(ts, w[1], w[4], fu, R(2) * sai[1], LO(IBIS_P_CANOPY_RLIQS), wk.ldr(WR_FVAPS), R(P.chs), dtime, ta, fvapa,
            fsena, P);
-  steph2o2(tl, w[2], w[5], (R(1) - fi0) * fl, R(2) * (lai[0] + sai[0]), fw2.rliql, R(tv.fvaplw),
+  steph2o2(tl, w[2], w[5], (R(1) - fi0) * fl, R(2) * (lai[0] + sai[0]), LO(IBIS_P_CANOPY_RLIQL), wk.ldr(WR_FVAPLW),
            R(P.chl), dtime, ta, fvapa, fsena, P);
   ST(IBIS_P_DIAG_FSENA, fsena);
   ST(IBIS_P_DIAG_FVAPA, fvapa);
@@ -374,21 +597,21 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
 
   // ---- 9. surface fluxes with the flux clamp (loops.py:665-674) ------------------------------
   const R fm = R(P.flux_max);
-  const R heatg = rad.solg + clip(R(tv.firg), -fm, fm) - clip(R(tv.fseng), -fm, fm) -
-                  hvasug * R(tv.fvapg);
-  const R heati = rad.soli + clip(R(tv.firi), -fm, fm) - clip(R(tv.fseni), -fm, fm) -
-                  hvasui * R(tv.fvapi);
+  const R heatg = LO(IBIS_P_CANOPY_SOLG) + clip(LO(IBIS_P_CANOPY_FIRG), -fm, fm) - clip(LO(IBIS_P_CANOPY_FSENG), -fm, fm) -
+                  hvasug * fvapg;
+  const R heati = LO(IBIS_P_CANOPY_SOLI) + clip(LO(IBIS_P_CANOPY_FIRI), -fm, fm) - clip(LO(IBIS_P_CANOPY_FSENI), -fm, fm) -
+                  hvasui * fvapi;
   ST(IBIS_P_SNOW_HEATI, heati);
 
   // ---- 10. snow (loops.py:679-848) ----------------------------------------------------------
   const R rhos = R(P.rhos), cice = R(P.cice), ch2o = R(P.ch2o), hfus = R(P.hfus);
   const R fhtop_snow = heati +
-                       cas.rainl * (ch2o * (cas.trainl - tmelt) + hfus + cice * (tmelt - tsno[0])) +
-                       cas.snowl * cice * (cas.tsnowl - tsno[0]);
+                       rainl * (ch2o * (trainl - tmelt) + hfus + cice * (tmelt - tsno[0])) +
+                       snowl * cice * (tsnowl - tsno[0]);
   ST(IBIS_P_HYDROLOGY_FHTOP_SNOW, fhtop_snow);
   const R xlb = fl * R(2) * (lai[0] + sai[0]);
   R hs[3] = {hsno[0], hsno[1], hsno[2]};
-  if (fi0 > R(0)) hs[0] = hsno[0] + (cas.rainl + cas.snowl - R(tv.fvapi)) * (dtime / rhos);
+  if (fi0 > R(0)) hs[0] = hsno[0] + (rainl + snowl - fvapi) * (dtime / rhos);
   // snowheat (snow_physics.py:109-229), solved in D
   R tsn[3];
   R tlsub_h;
@@ -436,12 +659,12 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
   const R hsnomin = R(P.hsnomin), hsnotop = R(P.hsnotop);
   R ht = hs[0] + hs[1] + hs[2];
   const R hts = ht > R(0) ? ht : hsnomin;
-  R dfi = (R(1) - fi0) * dtime * cas.snowg / (rhos * hts);
+  R dfi = (R(1) - fi0) * dtime * snowg / (rhos * hts);
   dfi = mn(dfi, R(1) - fi0);
   const R fi1 = fi0 + dfi;
   if (fi1 > R(0)) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) tsn[k] = (tsn[k] * fi0 + cas.tsnowg * dfi) / mx(fi1, eps);
+    for (int k = 0; k < 3; ++k) tsn[k] = (tsn[k] * fi0 + tsnowg * dfi) / mx(fi1, eps);
   }
   if (fi0 < eps && dfi > R(0)) {
     hs[0] = hsnotop;
@@ -550,13 +773,13 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
   const R rdt = R(1) / dtime;
   const R heatg_avg = (R(1) - fi0) * heatg + fi0 * R(sflo_bot) + zheat * rdt;
   ST(IBIS_P_SOIL_HEATG, heatg_avg);
-  const R fvapg_avg = clip((R(1) - fi0) * R(tv.fvapg), R(-1.0e-3), R(1.0e-3));
-  const R raing_nf = (R(1) - fi0) * cas.raing;
+  const R fvapg_avg = clip((R(1) - fi0) * fvapg, R(-1.0e-3), R(1.0e-3));
+  const R raing_nf = (R(1) - fi0) * raing;
   const R smelt_rate = smelt * rdt;
   const R r_un = raing_nf + smelt_rate;
   const bool rzero = r_un < eps;
   const R raing_avg = rzero ? R(0) : r_un;
-  const R traing_avg = rzero ? R(0) : (raing_nf * cas.traing + smelt_rate * tmelt) / mx(r_un, eps);
+  const R traing_avg = rzero ? R(0) : (raing_nf * traing + smelt_rate * tmelt) / mx(r_un, eps);
   ST(IBIS_P_HYDROLOGY_RAING_AVG, raing_avg);
   ST(IBIS_P_HYDROLOGY_TRAING_AVG, traing_avg);
 
@@ -610,8 +833,8 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
     for (int k = 0; k < NS; ++k) porosflo[k] = mx(poD[k] * mx(D(1) - wiD[k], epsD), epsD);
 
     // transpiration extraction per layer (loops.py:990-1048)
-    const D gtransu = D(tv.fvaput) * D(2) * D(lai[1]) * D(fu);
-    const D gtransl = D(tv.fvaplt) * D(2) * D(lai[0]) * D(fl) * (D(1) - D(fi0));
+    const D gtransu = wk.ldd(WD_FVAPUT) * D(2) * D(lai[1]) * D(fu);
+    const D gtransl = wk.ldd(WD_FVAPLT) * D(2) * D(lai[0]) * D(fl) * (D(1) - D(fi0));
     ST(IBIS_P_HYDROLOGY_GTRANSU, gtransu);
     ST(IBIS_P_HYDROLOGY_GTRANSL, gtransl);
     const D sstu = mx(D(stresstu), epsD), sstl = mx(D(stresstl), epsD);
@@ -916,7 +1139,17 @@ IBIS_HD void step_cell(const Cell<S>& c, const R vrow[IBIS_NPFT], const IbisPara
   for (int k = 0; k < 6; ++k) ST(IBIS_P_CANOPY_WLIQU + k, w[k]);
 }
 
+// The three phases in order for one cell (host builds; the GPU runs them as three kernels).
+template <class S, class R, class D>
+IBIS_HD void step_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
+                       const IbisParams& P, const StepConst& K) {
+  surface_cell<S, R, D>(c, wk, P, K);
+  canopy_cell<S, R, D>(c, wk, vrow, P, K);
+  ground_cell<S, R, D>(c, wk, P, K);
+}
+
 #undef LD
 #undef ST
+#undef LO
 
 }  // namespace ibis

@@ -70,6 +70,7 @@ class IBISState:
         self.nytimes = float(nytimes)
         self.snow_const = snow_const or SnowConstants()
         self.bedrock = bool(bedrock)
+        self._work = {}
 
     # -- sub-states -------------------------------------------------------------------
     def __getattr__(self, name):

@@ -59,6 +59,21 @@ def _precision(state: IBISState, precision):
     return code
 
 
+def workspace(state: IBISState, code: int, stream) -> torch.Tensor:
+    """The step's phase workspace for launches from ``state`` on ``stream`` (allocated once
+    per state, stream and size; launches on one stream are ordered, so they can share it)."""
+    lib = _lib.load()
+    nbytes = int(lib.ibis_step_workspace_bytes(C.c_int64(state.pitch), C.c_int(code)))
+    if nbytes < 0:
+        raise ValueError("bad pitch or precision for the step workspace")
+    key = (int(stream.cuda_stream), nbytes)
+    ws = state._work.get(key)
+    if ws is None:
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=state.device)
+        state._work[key] = ws
+    return ws
+
+
 def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: float,
            precision=None, copy_in=False, stream=None, cparams=None):
     """Enqueue one hour on ``stream`` (default: torch's current stream of the device)."""
@@ -82,7 +97,9 @@ def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: 
     if stream is None:
         stream = torch.cuda.current_stream(state_in.device)
     slot = state_in.slot_type
-    rc = lib.ibis_step(C.byref(ain), C.byref(aout),
+    ws = workspace(state_in, code, stream)
+    rc = lib.ibis_step(C.byref(ain), C.byref(aout), C.c_void_p(ws.data_ptr()),
+                       C.c_int64(ws.numel()),
                        C.c_void_p(slot.data_ptr() if slot is not None else 0),
                        C.c_int64(state_in.pitch), C.c_void_p(table.data_ptr()), C.byref(p),
                        C.c_double(dtime), C.c_double(state_in.nytimes),

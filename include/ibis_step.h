@@ -88,19 +88,32 @@ typedef struct IbisParams {
 #define IBIS_FLAG_COPY_IN 0x20
 
 #define IBIS_OK 0
-#define IBIS_E_ARG (-1)      /* null pointer, negative size, pitch < ncell */
+#define IBIS_E_ARG (-1)      /* null pointer, negative size, pitch < ncell, small workspace */
 #define IBIS_E_FLAGS (-2)    /* unknown precision or flag bits */
 #define IBIS_E_LAUNCH (-3)   /* the HIP launch failed */
 #define IBIS_E_BEDROCK (-4)  /* bedrock layer count / thicknesses inconsistent */
 
+/* Bytes of device workspace ibis_step needs for arenas of `pitch` cells at the precision
+ * selected by `flags` (the values handed between the step's phases that are not state
+ * fields: 34 planes in the solve precision, 25 in the bulk precision).  Negative on bad
+ * arguments.  No counterpart in the reference (its JAX step keeps these in XLA buffers). */
+int64_t ibis_step_workspace_bytes(int64_t pitch, int flags);
+
 /* One hour for `ncell` cells: reads the planes of `in`, writes every IO/OUT plane of `out`
- * (out may equal in).  slot_type: IBIS_NPFT int8 planes of `slot_pitch` (NULL: every slot
- * generic).  type_table: IBIS_TYPE_ROWS x IBIS_TYPE_COLS doubles, device memory.
+ * (out may equal in).  The step runs as three kernels on `hip_stream` (surface: radiation,
+ * interception, aerodynamics; canopy: the three energy-balance passes; ground: snow, soil,
+ * fire) that pass intermediates through `workspace` (device memory of at least
+ * ibis_step_workspace_bytes(pitch, flags) bytes, 256-byte aligned; its contents need no
+ * initialisation and are meaningless afterwards).  Calls that share a workspace must be
+ * ordered on one stream.  slot_type: IBIS_NPFT int8 planes of `slot_pitch` (NULL: every
+ * slot generic).  type_table: IBIS_TYPE_ROWS x IBIS_TYPE_COLS doubles, device memory.
  * nytimes: VegetationState.nytimes before the step (the step advances it by one).
+ * Allocates nothing and does not synchronise.
  * Replaces timestep_body (python-port/ibis/timestepping/loops.py:234-1284). */
-int ibis_step(const IbisArena* in, const IbisArena* out, const int8_t* slot_type,
-              int64_t slot_pitch, const double* type_table, const IbisParams* params,
-              double dtime, double nytimes, int64_t ncell, int flags, void* hip_stream);
+int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
+              int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
+              const double* type_table, const IbisParams* params, double dtime, double nytimes,
+              int64_t ncell, int flags, void* hip_stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ibis_last_error(void);

@@ -1,9 +1,17 @@
-"""Shared builders for the step tests: synthetic cells, parameters, oracle runs."""
+"""Shared builders for the step tests: synthetic cells, parameters, oracle runs, the host
+build of the kernel source."""
+import ctypes as C
+import os
+
 import numpy as np
 
-from ibis_amd import layout as L, synth
-from ibis_amd.params import default_sim_params, get_constants
+from ibis_amd import abi, layout as L, synth
+from ibis_amd.params import SnowConstants, default_sim_params, get_constants
 from oracle import compare, params as OP, run as orun
+
+HOSTCHECK = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                         "ibis_amd", "libibis_hostcheck.so")
+PREC_CODE = {"f64": abi.PREC_F64, "mixed": abi.PREC_MIXED, "f32": abi.PREC_F32}
 
 OUT_KEYS = [f.key for f in L.FIELDS if f.role != "in"]
 #: relative perturbation of the delta band and tolerance per precision: the FP64 kernel is
@@ -20,7 +28,16 @@ def workload(n, seed=11, hour=13, prec="f64", frozen=0.0, saturated=0.0):
     return f, slot
 
 
-def oracle_with_band(f, slot, prec, bedrock=True, consts=None, params=None):
+def glob_of(hsoi=None, froot=None, co2conc=380.0e-6, nytimes=0.0, snow=None):
+    """The leaves of a case that are not per cell, with the synthetic workload's defaults."""
+    return dict(hsoi=synth.HSOI if hsoi is None else np.asarray(hsoi, np.float64),
+                froot=OP.root_fractions() if froot is None else np.asarray(froot, np.float64),
+                co2conc=float(co2conc), nytimes=float(nytimes),
+                snow=snow if snow is not None else SnowConstants())
+
+
+def oracle_with_band(f, slot, prec, bedrock=True, consts=None, params=None, dtime=3600.0,
+                     glob=None):
     """Oracle outputs and delta band for a workload.
 
     Returns ``(band, refs, keys, tol)``: ``refs[0]`` is the plain oracle run, the rest are
@@ -28,15 +45,38 @@ def oracle_with_band(f, slot, prec, bedrock=True, consts=None, params=None):
     passes if it matches any of them."""
     consts = consts or get_constants()
     params = params or default_sim_params(bedrock=bedrock)
+    g = glob or glob_of()
     rel, tol = PREC[prec]
 
     def run(fl, sides=None):
-        return orun.run_step(fl, slot, consts, params, hsoi=synth.HSOI,
-                             froot=OP.root_fractions(), bedrock=bedrock, knife_sides=sides)
+        return orun.run_step(fl, slot, consts, params, dtime=dtime, hsoi=g["hsoi"],
+                             froot=g["froot"], co2conc=g["co2conc"], nytimes=g["nytimes"],
+                             snow=g["snow"], bedrock=bedrock, knife_sides=sides)
     keys = [k for k in OUT_KEYS if bedrock or k != "soil.tsoi_deep"]
     band, ref = compare.delta_band(run, f, keys, rel, tol)
     variants = compare.branch_variants(lambda sides: run(f, sides), ref)
     return band, [ref] + variants, keys, tol
+
+
+def host_step(f, slot, prec, bedrock=True, consts=None, params=None, dtime=3600.0, glob=None):
+    """One hour of the kernel source built for the host (``libibis_hostcheck.so``)."""
+    lib = C.CDLL(HOSTCHECK)
+    n = len(f["soil.tg"])
+    consts = consts or get_constants()
+    params = params or default_sim_params(bedrock=bedrock)
+    g = glob or glob_of()
+    p = abi.params_from(consts, params, g["snow"], g["hsoi"], g["froot"], g["co2conc"])
+    a = abi.pack(f, n, dtype=np.float64 if prec == "f64" else np.float32)
+    o = a.copy()
+    s = abi.pack_slots(slot, n)
+    tab = synth.type_table(params.stomata_params.vmax_pft)
+    flags = PREC_CODE[prec] | (abi.FLAG_BEDROCK if bedrock else 0)
+    rc = lib.ibis_hostcheck_step(a.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p),
+                                 C.c_int64(n), C.c_int64(n), s.ctypes.data_as(C.c_void_p),
+                                 C.c_int64(n), tab.ctypes.data_as(C.c_void_p), C.byref(p),
+                                 C.c_double(dtime), C.c_double(g["nytimes"]), C.c_int(flags))
+    assert rc == 0
+    return abi.unpack(o, n)
 
 
 def failing_cells(got, refs, keys, tol):
@@ -75,3 +115,54 @@ def assert_parity(got, refs, band, keys, prec, label=""):
         assert not (failing_cells(got, refs, keys, 1e-3) & ~band).any()
     assert bad.mean() < 0.005, bad.mean()
     return bad
+
+
+SITE_NPZ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "site_transient_1996_step.npz")
+
+
+def site_case(daytime_hour=None, seed=5):
+    """The reference's saved 171-cell site state (real IBIS state, float32) as step inputs.
+
+    Its own stored forcing is a night hour (coszen = 0 in every cell); ``daytime_hour`` swaps
+    in the synthetic forcing of that hour (latitudes drawn uniform over the benchmark range)
+    to exercise the solar path on the same state.  Slots are generic (-1): the saved state
+    does not carry the slot type ids.  Returns ``(fields, slot, glob)``."""
+    z = np.load(SITE_NPZ, allow_pickle=False)
+    f = {k: z[k].astype(np.float64) for k in z.files if not k.startswith("glob.")}
+    n = len(f["soil.tg"])
+    snow = SnowConstants(**{k: float(z[f"glob.snow.{k}"]) for k in
+                            ("z0sno", "rhos", "consno", "hsnotop", "hsnomin", "fimin", "fimax")})
+    glob = glob_of(hsoi=z["glob.soil.hsoi"], froot=z["glob.vegetation.froot"],
+                   co2conc=float(z["glob.atm.co2conc"]),
+                   nytimes=float(z["glob.vegetation.nytimes"]), snow=snow)
+    if daytime_hour is not None:
+        lat = -56.0 + 140.0 * synth.uniform(seed, 99, np.arange(n, dtype=np.uint64), 0)
+        forc = synth.forcing(n, seed, daytime_hour, lat, f["atm.psurf"])
+        f.update({k: np.asarray(v, np.float32).astype(np.float64) for k, v in forc.items()})
+    slot = np.full((n, 12), -1, dtype=np.int8)
+    return f, slot, glob
+
+
+def column_water(fields, hsoi, rhow=1000.0):
+    """Soil column plus puddle water, kg m-2 (``tests/test_water_balance.py:22-27``)."""
+    hs = np.asarray(hsoi, np.float64)[:6]
+    w = rhow * fields["soil.poros"] * hs[None, :] * (
+        (1.0 - fields["soil.wisoi"]) * fields["soil.wsoi"] + fields["soil.wisoi"])
+    return w.sum(axis=1) + fields["soil.wpud"] + fields["soil.wipud"]
+
+
+def water_residual(before, after, hsoi, dtime):
+    """d(column water) - (rain reaching the soil - soil evaporation - transpiration - runoff -
+    drainage) dt, and the flux scale the reference judges it by
+    (``tests/test_water_balance.py:38-47``)."""
+    into = after["hydrology.raing_avg"]
+    out = (after["hydrology.gsuvap"] + after["hydrology.gtrans"] + after["hydrology.grunof"]
+           + after["hydrology.gdrain"])
+    w0 = column_water(before, hsoi)
+    merged = dict(before)
+    merged.update({k: v for k, v in after.items() if not k.startswith("_")})
+    w1 = column_water(merged, hsoi)
+    resid = (w1 - w0) - (into - out) * dtime
+    scale = np.maximum(np.abs(into) + np.abs(out), 1e-6) * dtime
+    return resid, scale

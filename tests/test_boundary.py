@@ -28,7 +28,7 @@ def test_io_contract_matches_the_reference_leaf_count():
 def test_library_exports_every_declared_symbol():
     lib = _lib.load()
     with open(os.path.join(ROOT, "include", "ibis_step.h")) as fh:
-        decl = re.findall(r"^(?:int|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
+        decl = re.findall(r"^(?:int|int64_t|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
     assert set(decl) == set(_lib.EXPORTS)
     for name in decl:
         assert hasattr(lib, name)
@@ -41,11 +41,13 @@ def test_params_struct_matches_the_c_layout():
     assert abi.IbisParams.co2conc.offset == C.sizeof(abi.IbisParams) - 8
 
 
-def _call(lib, ain, aout, n, flags, p=None, table=1, dtime=3600.0):
+def _call(lib, ain, aout, n, flags, p=None, table=1, dtime=3600.0, ws=256, ws_bytes=1 << 20):
+    # host-side argument checks only: the fake device addresses are never dereferenced
     p = p if p is not None else abi.IbisParams()
-    return lib.ibis_step(C.byref(ain) if ain else None, C.byref(aout) if aout else None, None,
-                         C.c_int64(0), C.c_void_p(table), C.byref(p), C.c_double(dtime),
-                         C.c_double(0.0), C.c_int64(n), C.c_int(flags), None)
+    return lib.ibis_step(C.byref(ain) if ain else None, C.byref(aout) if aout else None,
+                         C.c_void_p(ws), C.c_int64(ws_bytes), None, C.c_int64(0),
+                         C.c_void_p(table), C.byref(p), C.c_double(dtime), C.c_double(0.0),
+                         C.c_int64(n), C.c_int(flags), None)
 
 
 def test_argument_errors_return_codes_without_touching_the_gpu():
@@ -60,7 +62,20 @@ def test_argument_errors_return_codes_without_touching_the_gpu():
     p = abi.IbisParams()
     assert _call(lib, a, a, 4, abi.FLAG_BEDROCK, p=p) == -4   # bedrock thicknesses unset
     assert b"bedrock" in lib.ibis_last_error()
+    assert _call(lib, a, a, 4, 0, ws=None) == -1          # no workspace
+    assert _call(lib, a, a, 4, 0, ws_bytes=16) == -1      # workspace too small
+    assert _call(lib, a, a, 4, 0, ws=264) == -1           # workspace misaligned
+    assert b"workspace" in lib.ibis_last_error()
     assert _call(lib, a, a, 0, 0) == 0                    # nothing to do
+
+
+def test_workspace_size_per_precision():
+    lib = _lib.load()
+    # 34 planes in the solve precision, 25 in the bulk precision
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_MIXED) == 100 * (34 * 8 + 25 * 4)
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F32) == 100 * 59 * 4
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F64) == 100 * 59 * 8
+    assert lib.ibis_step_workspace_bytes(-1, 0) < 0
 
 
 def test_product_defaults_equal_the_reference_defaults():

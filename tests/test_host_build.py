@@ -4,39 +4,10 @@
 HIP kernel runs -- with g++.  It checks the kernel's logic on the CPU; the GPU tests
 (test_gpu_step.py) check the compiled kernel itself.
 """
-import ctypes as C
-import os
-
 import numpy as np
 import pytest
 
-from ibis_amd import abi, synth
-from ibis_amd.params import SnowConstants, default_sim_params, get_constants
-from oracle import compare, params as OP
-
-from _helpers import assert_parity, failing_cells, oracle_with_band, workload
-
-LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ibis_amd",
-                   "libibis_hostcheck.so")
-CODE = {"f64": abi.PREC_F64, "mixed": abi.PREC_MIXED, "f32": abi.PREC_F32}
-
-
-def host_step(f, slot, prec, bedrock=True, nytimes=0.0):
-    lib = C.CDLL(LIB)
-    n = len(f["soil.tg"])
-    consts, params = get_constants(), default_sim_params(bedrock=bedrock)
-    p = abi.params_from(consts, params, SnowConstants(), synth.HSOI, OP.root_fractions(), 380e-6)
-    a = abi.pack(f, n, dtype=np.float64 if prec == "f64" else np.float32)
-    o = a.copy()
-    s = abi.pack_slots(slot, n)
-    tab = synth.type_table(params.stomata_params.vmax_pft)
-    flags = CODE[prec] | (abi.FLAG_BEDROCK if bedrock else 0)
-    rc = lib.ibis_hostcheck_step(a.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p),
-                                 C.c_int64(n), C.c_int64(n), s.ctypes.data_as(C.c_void_p),
-                                 C.c_int64(n), tab.ctypes.data_as(C.c_void_p), C.byref(p),
-                                 C.c_double(3600.0), C.c_double(nytimes), C.c_int(flags))
-    assert rc == 0
-    return abi.unpack(o, n)
+from _helpers import assert_parity, host_step, oracle_with_band, workload
 
 
 @pytest.mark.parametrize("prec", ["f64", "mixed"])
