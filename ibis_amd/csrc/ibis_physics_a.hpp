@@ -7,7 +7,7 @@
 #pragma once
 
 #include "ibis_math.hpp"
-#include "../../include/ibis_step.h"
+#include "ibis_step.h"
 
 namespace ibis {
 

@@ -234,16 +234,20 @@ template <class D> IBIS_HD IBIS_INLINE void gauss9(D A[9][9], D b[9], D x[9]) {
   }
 }
 
-// Inputs of turvap that stay fixed over the three passes.
-template <class D> struct TvFix {
-  D tuold, tsold, tlold, tgold, tiold;
-  D cp, tfac, xu, xs, xl, fi, fu, fl, lai0, lai1, sai0, sai1;
-  D frac[12];
-  D chux, chsx, chlx, w[6];
-  D rliqu, rliqs, rliql, fwetu, fwets, fwetl;
-  D cog, coi, zirg, ziri, wpud, wipud, wisoi1, wsoi1, swilt1, suction1, bex1, hvasug, hvasui;
-  D tsoi1, tsno1, solu, sols, soll, solg, soli, firu, firs, firl, firg, firi, firb;
-  D pfluxu, pfluxs, pfluxl, ta, qa, psurf;
+// Inputs of turvap that stay fixed over the three passes.  Values computed in the bulk
+// precision are kept in it (R) and widened at each use, which is exact; only the aerodynamic
+// heat capacity, the host's tfac and the infrared terms are solve-precision values.
+template <class R, class D> struct TvFix {
+  R tuold, tsold, tlold, tgold, tiold;
+  D cp, tfac;
+  R xu, xs, xl, fi, fu, fl, lai0, lai1, sai0, sai1;
+  R frac[12];
+  R chux, chsx, chlx, w[6];
+  R rliqu, rliqs, rliql, fwetu, fwets, fwetl;
+  R cog, coi, zirg, ziri, wpud, wipud, wisoi1, wsoi1, swilt1, suction1, bex1, hvasug, hvasui;
+  R tsoi1, tsno1, solu, sols, soll, solg, soli;
+  D firu, firs, firl, firg, firi, firb;
+  R pfluxu, pfluxs, pfluxl, ta, qa, psurf;
 };
 
 template <class D> struct TvOut {
@@ -254,18 +258,18 @@ template <class D> struct TvOut {
 };
 
 // turvap (canopy_exchange.py:470-811)
-template <class D>
-IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, D su, D ss, D sl,
+template <class R, class D>
+IBIS_HD void turvap(TvOut<D>& o, const TvFix<R, D>& f, const D cur[9], D cu, D cl, D su, D ss, D sl,
                     D sg, D si, const D tc[5], D wu, D ws, D wl, D wg, D wi, D dtime,
                     const IbisParams& P) {
   const D eps = D(P.epsilon), tmelt = D(P.tmelt);
-  const D tu = wu * cur[0] + (D(1) - wu) * f.tuold;
-  const D ts = ws * cur[1] + (D(1) - ws) * f.tsold;
-  const D tl = wl * cur[2] + (D(1) - wl) * f.tlold;
+  const D tu = wu * cur[0] + (D(1) - wu) * D(f.tuold);
+  const D ts = ws * cur[1] + (D(1) - ws) * D(f.tsold);
+  const D tl = wl * cur[2] + (D(1) - wl) * D(f.tlold);
   const D t34 = cur[4], q12 = cur[5], q34 = cur[6];
-  const D tg = wg * cur[7] + (D(1) - wg) * f.tgold;
-  const D ti = wi * cur[8] + (D(1) - wi) * f.tiold;
-  const D ps = f.psurf;
+  const D tg = wg * cur[7] + (D(1) - wg) * D(f.tgold);
+  const D ti = wi * cur[8] + (D(1) - wi) * D(f.tiold);
+  const D ps = D(f.psurf);
   D e = esat(tu);
   const D qu = qsat(e, ps), dqu = mn(dqsat(tu, qu, e), qu * D(0.1));
   e = esat(ts);
@@ -277,30 +281,30 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
   e = esat(ti);
   const D qi = qsat(e, ps), dqi = mn(dqsat(ti, qi, e), qi * D(0.1));
 
-  const D zwpud = clip(D(0.5) * (f.wpud + f.wipud) / mx(D(P.wpudmax), eps), D(0), D(0.5));
-  const D zwsoi = (D(1) - f.wisoi1) * f.wsoi1 + f.wisoi1;
+  const D zwpud = clip(D(0.5) * (D(f.wpud) + D(f.wipud)) / mx(D(P.wpudmax), eps), D(0), D(0.5));
+  const D zwsoi = (D(1) - D(f.wisoi1)) * D(f.wsoi1) + D(f.wisoi1);
   const D zwtot = zwpud + (D(1) - zwpud) * zwsoi;
-  const D psig = -D(P.grav) * f.suction1 * r_pow(zwtot, -f.bex1);
+  const D psig = -D(P.grav) * D(f.suction1) * r_pow(zwtot, -D(f.bex1));
   const D hfac = r_exp(psig / (D(P.rvap) * tg));
   const D hfac2 = q34 / mx(qg, eps);
-  D betaw = clip((zwtot - f.swilt1) / mx(D(1) - f.swilt1, eps), D(0), D(1));
+  D betaw = clip((zwtot - D(f.swilt1)) / mx(D(1) - D(f.swilt1), eps), D(0), D(1));
   if (P.soil_beta_power != 1.0) betaw = r_pow(betaw, D(P.soil_beta_power));
   const D e34 = esat(t34);
   const D qs1 = qsat(e34, ps);
   const D dqs1 = dqsat(t34, qs1, e34);
-  const D betafac = D(P.hvap) * dqs1 / (f.cp + (D(4.0 * 0.95) * D(P.stef) * cube(t34)) / sg);
+  const D betafac = D(P.hvap) * dqs1 / (D(f.cp) + (D(4.0 * 0.95) * D(P.stef) * cube(t34)) / sg);
   const D betas = betaw / (D(1) + betafac * (D(1) - betaw));
   const D qgfac0 = (betas > D(0)) ? betas * hfac + (D(1) - betas) * hfac2 : hfac2;
   const D qeps = D(1.0e-10);
   const D qgfac = (q34 > qg + qeps) ? D(1) : qgfac0;
-  const D fwetux = (q12 > qu + qeps) ? D(1) : f.fwetu;
-  const D fwetsx = (q12 > qs + qeps) ? D(1) : f.fwets;
-  const D fwetlx = (q34 > ql + qeps) ? D(1) : f.fwetl;
-  const D fradu = (f.lai1 > eps) ? (f.solu + f.firu) / (D(2) * f.lai1) : D(0);
-  const D frads = (f.sai1 > eps) ? (f.sols + f.firs) / (D(2) * f.sai1) : D(0);
-  const D fradl = ((f.lai0 + f.sai0) > eps) ? (f.soll + f.firl) / (D(2) * (f.lai0 + f.sai0)) : D(0);
+  const D fwetux = (q12 > qu + qeps) ? D(1) : D(f.fwetu);
+  const D fwetsx = (q12 > qs + qeps) ? D(1) : D(f.fwets);
+  const D fwetlx = (q34 > ql + qeps) ? D(1) : D(f.fwetl);
+  const D fradu = (D(f.lai1) > eps) ? (D(f.solu) + D(f.firu)) / (D(2) * D(f.lai1)) : D(0);
+  const D frads = (D(f.sai1) > eps) ? (D(f.sols) + D(f.firs)) / (D(2) * D(f.sai1)) : D(0);
+  const D fradl = ((D(f.lai0) + D(f.sai0)) > eps) ? (D(f.soll) + D(f.firl)) / (D(2) * (D(f.lai0) + D(f.sai0))) : D(0);
 
-  const D ta = f.ta;
+  const D ta = D(f.ta);
   const D hvc = D(P.hvap), hsc = D(P.hsub), cvc = D(P.cvap), chc = D(P.ch2o), cic = D(P.cice);
   const D hvu = hvc + cvc * (ta - tmelt) - chc * (tu - tmelt);
   const D hsu = hsc + cvc * (ta - tmelt) - cic * (tu - tmelt);
@@ -308,20 +312,22 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
   const D hss = hsc + cvc * (ta - tmelt) - cic * (ts - tmelt);
   const D hvl = hvc + cvc * (ta - tmelt) - chc * (tl - tmelt);
   const D hsl = hsc + cvc * (ta - tmelt) - cic * (tl - tmelt);
-  const D* fr = f.frac;
+  D fr[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) fr[j] = D(f.frac[j]);
   const D tub = tc[0], tuc = tc[1], tls = tc[2], tl3 = tc[3], tl4 = tc[4];
-  const D suw = mn(fwetux * su, D(0.8) * (f.w[0] + f.w[3]) / mx(dtime * (qu - q12), eps));
+  const D suw = mn(fwetux * su, D(0.8) * (D(f.w[0]) + D(f.w[3])) / mx(dtime * (qu - q12), eps));
   const D sut = mx(D(0), (D(1) - fwetux) * D(0.5) *
                               (tub * fr[0] + tub * fr[1] + tub * fr[2] + tuc * fr[3] +
                                tub * fr[4] + tuc * fr[5] + tub * fr[6] + tub * fr[7]));
-  const D suh = suw * (f.rliqu * hvu + (D(1) - f.rliqu) * hsu) + sut * hvu;
-  const D ssw = mn(fwetsx * ss, D(0.8) * (f.w[1] + f.w[4]) / mx(dtime * (qs - q12), eps));
-  const D ssh = ssw * (f.rliqs * hvs + (D(1) - f.rliqs) * hss);
-  const D slw = mn(fwetlx * sl, D(0.8) * (f.w[2] + f.w[5]) / mx(dtime * (ql - q34), eps));
+  const D suh = suw * (D(f.rliqu) * hvu + (D(1) - D(f.rliqu)) * hsu) + sut * hvu;
+  const D ssw = mn(fwetsx * ss, D(0.8) * (D(f.w[1]) + D(f.w[4])) / mx(dtime * (qs - q12), eps));
+  const D ssh = ssw * (D(f.rliqs) * hvs + (D(1) - D(f.rliqs)) * hss);
+  const D slw = mn(fwetlx * sl, D(0.8) * (D(f.w[2]) + D(f.w[5])) / mx(dtime * (ql - q34), eps));
   const D slt0 = mx(D(0), (D(1) - fwetlx) * D(0.5) *
                               (tls * fr[8] + tls * fr[9] + tl4 * fr[10] + tl3 * fr[11]));
-  const D slt = slt0 * f.lai0 / mx(f.lai0 + f.sai0, eps);
-  const D slh = slw * (f.rliql * hvl + (D(1) - f.rliql) * hsl) + slt * hvl;
+  const D slt = slt0 * D(f.lai0) / mx(D(f.lai0) + D(f.sai0), eps);
+  const D slh = slw * (D(f.rliql) * hvl + (D(1) - D(f.rliql)) * hsl) + slt * hvl;
 
   D A[9][9];
   D b[9];
@@ -332,31 +338,31 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
     b[i] = D(0);
   }
   const D rw = D(1) / dtime;
-  const D cp = f.cp, fi = f.fi, xu = f.xu, xs = f.xs, xl = f.xl;
+  const D cp = D(f.cp), fi = D(f.fi), xu = D(f.xu), xs = D(f.xs), xl = D(f.xl);
   D w2 = su * cp;
-  A[0][0] = f.chux * rw + wu * w2 + wu * suh * dqu;
+  A[0][0] = D(f.chux) * rw + wu * w2 + wu * suh * dqu;
   A[0][3] = -w2;
   A[0][5] = -suh;
-  b[0] = f.tuold * f.chux * rw - (D(1) - wu) * w2 * tu - suh * (qu - wu * dqu * tu) + fradu -
-         f.pfluxu;
+  b[0] = D(f.tuold) * D(f.chux) * rw - (D(1) - wu) * w2 * tu - suh * (qu - wu * dqu * tu) + fradu -
+         D(f.pfluxu);
   w2 = ss * cp;
-  A[1][1] = f.chsx * rw + ws * w2 + ws * ssh * dqs;
+  A[1][1] = D(f.chsx) * rw + ws * w2 + ws * ssh * dqs;
   A[1][3] = -w2;
   A[1][5] = -ssh;
-  b[1] = f.tsold * f.chsx * rw - (D(1) - ws) * w2 * ts - ssh * (qs - ws * dqs * ts) + frads -
-         f.pfluxs;
+  b[1] = D(f.tsold) * D(f.chsx) * rw - (D(1) - ws) * w2 * ts - ssh * (qs - ws * dqs * ts) + frads -
+         D(f.pfluxs);
   w2 = sl * cp;
-  A[2][2] = f.chlx * rw + wl * w2 + wl * slh * dql;
+  A[2][2] = D(f.chlx) * rw + wl * w2 + wl * slh * dql;
   A[2][4] = -w2;
   A[2][6] = -slh;
-  b[2] = f.tlold * f.chlx * rw - (D(1) - wl) * w2 * tl - slh * (ql - wl * dql * tl) + fradl -
-         f.pfluxl;
+  b[2] = D(f.tlold) * D(f.chlx) * rw - (D(1) - wl) * w2 * tl - slh * (ql - wl * dql * tl) + fradl -
+         D(f.pfluxl);
   const D ru = xu * su, rs = xs * ss;
   A[3][0] = -wu * ru;
   A[3][1] = -ws * rs;
   A[3][3] = cu + cl + ru + rs;
   A[3][4] = -cl;
-  b[3] = cu * ta * f.tfac + (D(1) - wu) * ru * tu + (D(1) - ws) * rs * ts;
+  b[3] = cu * ta * D(f.tfac) + (D(1) - wu) * ru * tu + (D(1) - ws) * rs * ts;
   const D rl = xl * sl, ri = fi * si;
   A[4][2] = -wl * rl;
   A[4][3] = -cl;
@@ -369,7 +375,7 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
   A[5][1] = -ws * rsq * dqs;
   A[5][5] = cu + cl + ruq + rsq;
   A[5][6] = -cl;
-  b[5] = cu * f.qa + ruq * (qu - wu * dqu * tu) + rsq * (qs - ws * dqs * ts);
+  b[5] = cu * D(f.qa) + ruq * (qu - wu * dqu * tu) + rsq * (qs - ws * dqs * ts);
   const D rlq = xl * (slw + slt), rgq = (D(1) - fi) * sg;
   A[6][2] = -wl * rlq * dql;
   A[6][5] = -cl;
@@ -378,40 +384,40 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
   A[6][8] = -wi * fi * si * dqi;
   b[6] = rlq * (ql - wl * dql * tl) + rgq * qgfac * (qg - wg * dqg * tg) +
          fi * si * (qi - wi * dqi * ti);
-  D a1 = sg * cp, a2 = sg * f.hvasug;
+  D a1 = sg * cp, a2 = sg * D(f.hvasug);
   A[7][4] = -a1;
   A[7][6] = -a2;
-  A[7][7] = a1 + a2 * qgfac * dqg + f.cog + f.zirg;
-  b[7] = -a2 * qgfac * (qg - dqg * tg) + f.cog * f.tsoi1 + f.solg + f.firg + f.zirg * f.tgold;
+  A[7][7] = a1 + a2 * qgfac * dqg + D(f.cog) + D(f.zirg);
+  b[7] = -a2 * qgfac * (qg - dqg * tg) + D(f.cog) * D(f.tsoi1) + D(f.solg) + D(f.firg) + D(f.zirg) * D(f.tgold);
   a1 = si * cp;
-  a2 = si * f.hvasui;
+  a2 = si * D(f.hvasui);
   A[8][4] = -a1;
   A[8][6] = -a2;
-  A[8][8] = a1 + a2 * dqi + f.coi + f.ziri;
-  b[8] = -a2 * (qi - dqi * ti) + f.coi * f.tsno1 + f.soli + f.firi + f.ziri * f.tiold;
+  A[8][8] = a1 + a2 * dqi + D(f.coi) + D(f.ziri);
+  b[8] = -a2 * (qi - dqi * ti) + D(f.coi) * D(f.tsno1) + D(f.soli) + D(f.firi) + D(f.ziri) * D(f.tiold);
   // bare-cell identity rows (canopy_exchange.py:709-747)
   if ((xu + xs + xl) < eps) {
 #pragma unroll
     for (int i = 0; i < 7; ++i)
 #pragma unroll
       for (int j = 0; j < 9; ++j) A[i][j] = (i == j) ? D(1) : D(0);
-    const D tt = ta * f.tfac;
+    const D tt = ta * D(f.tfac);
     b[0] = b[1] = b[2] = b[3] = b[4] = tt;
-    b[5] = b[6] = f.qa;
-    b[7] = b[7] + ((fradu - f.pfluxu) * xu + (frads - f.pfluxs) * xs + (fradl - f.pfluxl) * xl);
+    b[5] = b[6] = D(f.qa);
+    b[7] = b[7] + ((fradu - D(f.pfluxu)) * xu + (frads - D(f.pfluxs)) * xs + (fradl - D(f.pfluxl)) * xl);
   }
   IBIS_TRACE_SYSTEM(A, b);
   D x[9];
   gauss9(A, b, x);
 #pragma unroll
   for (int j = 0; j < 9; ++j) o.t[j] = x[j];
-  o.fsena = cp * cu * (ta * f.tfac - x[3]);
+  o.fsena = cp * cu * (ta * D(f.tfac) - x[3]);
   o.fsenu = cp * su * (wu * x[0] + (D(1) - wu) * tu - x[3]);
   o.fsens = cp * ss * (ws * x[1] + (D(1) - ws) * ts - x[3]);
   o.fsenl = cp * sl * (wl * x[2] + (D(1) - wl) * tl - x[3]);
   o.fseng = cp * sg * (wg * x[7] + (D(1) - wg) * tg - x[4]);
   o.fseni = cp * si * (wi * x[8] + (D(1) - wi) * ti - x[4]);
-  o.fvapa = cu * (f.qa - x[5]);
+  o.fvapa = cu * (D(f.qa) - x[5]);
   const D quav = qu + wu * dqu * (x[0] - tu);
   o.fvapuw = suw * (quav - x[5]);
   o.fvaput = mx(D(0), sut * (quav - x[5]));
@@ -424,13 +430,13 @@ IBIS_HD void turvap(TvOut<D>& o, const TvFix<D>& f, const D cur[9], D cu, D cl, 
   o.fvapg = sg * (qgfac * qgav - x[6]);
   const D qiav = qi + wi * dqi * (x[8] - ti);
   o.fvapi = si * (qiav - x[6]);
-  o.firg = f.firg - wg * f.zirg * (x[7] - f.tgold);
-  o.firi = f.firi - wi * f.ziri * (x[8] - f.tiold);
-  o.firb = f.firb + (D(1) - fi) * wg * f.zirg * (x[7] - f.tgold) +
-           fi * wi * f.ziri * (x[8] - f.tiold);
+  o.firg = D(f.firg) - wg * D(f.zirg) * (x[7] - D(f.tgold));
+  o.firi = D(f.firi) - wi * D(f.ziri) * (x[8] - D(f.tiold));
+  o.firb = D(f.firb) + (D(1) - fi) * wg * D(f.zirg) * (x[7] - D(f.tgold)) +
+           fi * wi * D(f.ziri) * (x[8] - D(f.tiold));
   o.ginvap = o.fvapuw * xu + o.fvaps * xs + o.fvaplw * xl;
   o.gsuvap = o.fvapg * (D(1) - fi) + o.fvapi * fi;
-  o.gtrans = o.fvaput * D(2) * f.lai1 * f.fu + o.fvaplt * D(2) * f.lai0 * f.fl * (D(1) - fi);
+  o.gtrans = o.fvaput * D(2) * D(f.lai1) * D(f.fu) + o.fvaplt * D(2) * D(f.lai0) * D(f.fl) * (D(1) - fi);
 }
 
 // ---------------------------------------------------------------------------------------

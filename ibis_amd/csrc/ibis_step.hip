@@ -17,6 +17,11 @@
 namespace ibis {
 
 constexpr int kBlock = 256;
+// Waves per SIMD each kernel is compiled for.  95,333 cells are 1,490 waves on 1,024 SIMDs,
+// so a kernel held to one wave per SIMD runs its waves in two rounds; surface and ground fit
+// two waves at the cost of a few spilled registers, which measures faster.  The canopy
+// kernel's three passes spill far more at two waves and stay at one.
+constexpr int kWavesSurface = 2, kWavesCanopy = 1, kWavesGround = 2;
 
 template <class R, class D>
 __device__ IBIS_INLINE Work<R, D> work_at(void* ws, int64_t pitch, int64_t i) {
@@ -26,9 +31,9 @@ __device__ IBIS_INLINE Work<R, D> work_at(void* ws, int64_t pitch, int64_t i) {
 }
 
 template <class S, class R, class D>
-__global__ __launch_bounds__(kBlock) void surface_kernel(const S* in, S* out, void* ws,
-                                                         int64_t pitch, int64_t ncell,
-                                                         IbisParams P, StepConst K, int copy_in) {
+__global__ __launch_bounds__(kBlock, kWavesSurface) void surface_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, IbisParams P, StepConst K,
+    int copy_in) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= ncell) return;
   if (copy_in) {
@@ -45,11 +50,9 @@ __global__ __launch_bounds__(kBlock) void surface_kernel(const S* in, S* out, vo
 }
 
 template <class S, class R, class D>
-__global__ __launch_bounds__(kBlock) void canopy_kernel(const S* in, S* out, void* ws,
-                                                        int64_t pitch, int64_t ncell,
-                                                        const int8_t* slot, int64_t slot_pitch,
-                                                        const double* table, IbisParams P,
-                                                        StepConst K) {
+__global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, const int8_t* slot,
+    int64_t slot_pitch, const double* table, IbisParams P, StepConst K) {
   __shared__ R trow[IBIS_TYPE_ROWS];
   for (int t = threadIdx.x; t < IBIS_TYPE_ROWS; t += blockDim.x) trow[t] = R(table[t]);
   __syncthreads();
@@ -67,9 +70,8 @@ __global__ __launch_bounds__(kBlock) void canopy_kernel(const S* in, S* out, voi
 }
 
 template <class S, class R, class D>
-__global__ __launch_bounds__(kBlock) void ground_kernel(const S* in, S* out, void* ws,
-                                                        int64_t pitch, int64_t ncell,
-                                                        IbisParams P, StepConst K) {
+__global__ __launch_bounds__(kBlock, kWavesGround) void ground_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, IbisParams P, StepConst K) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= ncell) return;
   const Cell<S> c{in, out, pitch, i};

@@ -180,10 +180,33 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisPara
 
   // ---- 2-5. wet fractions, radiation, cascade ----------------------------------------------
   const Fwet<R> fw = fwetcal(tu0, ts0, tl0, w0, P);
-  RadOut<R> rad;
-  radiation(rad, coszen, solad, solai, lai, sai, fu, fl, tu0, ts0, tl0, fi0, tsno[0], wsoi[0],
-            wisoi[0], LD(IBIS_P_SOIL_ALBSAV), LD(IBIS_P_SOIL_ALBSAN), fw, P);
+  // two-stream solar radiation in the solve precision: its closed form divides by
+  // sigma = p^2 - q^2, which the reference only keeps above ~0.002 p^2 (radiation.py:208-214),
+  // and takes canopy absorption as 1 - reflected - transmitted
   const D laiD[2] = {D(lai[0]), D(lai[1])}, saiD[2] = {D(sai[0]), D(sai[1])};
+  RadOut<R> rad;
+  {
+    const D soladD[2] = {D(solad[0]), D(solad[1])}, solaiD[2] = {D(solai[0]), D(solai[1])};
+    Fwet<D> fwD;
+    fwD.fwetu = fw.fwetu; fwD.fwets = fw.fwets; fwD.fwetl = fw.fwetl;
+    fwD.rliqu = fw.rliqu; fwD.rliqs = fw.rliqs; fwD.rliql = fw.rliql;
+    RadOut<D> rd;
+    radiation(rd, D(coszen), soladD, solaiD, laiD, saiD, D(fu), D(fl), D(tu0), D(ts0), D(tl0),
+              D(fi0), D(tsno[0]), D(wsoi[0]), D(wisoi[0]), D(LD(IBIS_P_SOIL_ALBSAV)),
+              D(LD(IBIS_P_SOIL_ALBSAN)), fwD, P);
+    rad.solu = R(rd.solu); rad.sols = R(rd.sols); rad.soll = R(rd.soll);
+    rad.solg = R(rd.solg); rad.soli = R(rd.soli);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) { rad.asurd[b] = R(rd.asurd[b]); rad.asuri[b] = R(rd.asuri[b]); }
+    rad.topparu = R(rd.topparu); rad.topparl = R(rd.topparl);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      rad.scalcoefu[j] = R(rd.scalcoefu[j]);
+      rad.scalcoefl[j] = R(rd.scalcoefl[j]);
+    }
+    rad.termu6 = R(rd.termu6); rad.termu7 = R(rd.termu7);
+    rad.terml6 = R(rd.terml6); rad.terml7 = R(rd.terml7);
+  }
   IrOut<D> ir;
   irrad(ir, D(fira), laiD, saiD, D(fu), D(fl), D(fi0), D(tu0), D(ts0), D(tl0), D(tg0), D(ti0), P);
   ST(IBIS_P_CANOPY_SOLU, rad.solu);
@@ -365,7 +388,7 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
 
   const R hsoi0 = R(P.hsoi[0]);
 
-  TvFix<D> F;
+  TvFix<R, D> F;
   {
     const R cog = consoi0 / mx(R(0.5) * hsoi0, eps);
     const R coi = R(P.consno) / mx(R(0.5) * mx(hsno0, R(P.hsnotop)), eps);
@@ -375,7 +398,7 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
     F.cp = aero.cp; F.tfac = D(K.tfac); F.xu = xu; F.xs = xs; F.xl = xl; F.fi = fi0;
     F.fu = fu; F.fl = fl; F.lai0 = lai[0]; F.lai1 = lai[1]; F.sai0 = sai[0]; F.sai1 = sai[1];
 #pragma unroll
-    for (int j = 0; j < IBIS_NPFT; ++j) F.frac[j] = D(LD(IBIS_P_VEGETATION_FRAC + j));
+    for (int j = 0; j < IBIS_NPFT; ++j) F.frac[j] = LD(IBIS_P_VEGETATION_FRAC + j);
     F.chux = R(P.chu) + R(P.ch2o) * casw[0] + R(P.cice) * casw[3];
     F.chsx = R(P.chs) + R(P.ch2o) * casw[1] + R(P.cice) * casw[4];
     F.chlx = R(P.chl) + R(P.ch2o) * casw[2] + R(P.cice) * casw[5];
@@ -445,11 +468,11 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
             psurf, rad, fw2.fwetu, fw2.fwetl, lai, sai, vub, vuc, vrow[8], vrow[11], vrow[10],
             stresstu, stresstl, co2, zw, P);
     if (!first) {
-      wp[0] = impexp(cur[0], F.chux, F.w[0], F.w[3], wp[0], P);
-      wp[1] = impexp(cur[1], F.chsx, F.w[1], F.w[4], wp[1], P);
-      wp[2] = impexp(cur[2], F.chlx, F.w[2], F.w[5], wp[2], P);
+      wp[0] = impexp(cur[0], D(F.chux), D(F.w[0]), D(F.w[3]), wp[0], P);
+      wp[1] = impexp(cur[1], D(F.chsx), D(F.w[1]), D(F.w[4]), wp[1], P);
+      wp[2] = impexp(cur[2], D(F.chlx), D(F.w[2]), D(F.w[5]), wp[2], P);
       wp[3] = impexp(cur[7], chgxD, wlgxD, wigxD, wp[3], P);
-      wp[4] = impexp2(cur[8], F.tiold, wp[4], P);
+      wp[4] = impexp2(cur[8], D(F.tiold), wp[4], P);
     }
     D tcD[5];
 #pragma unroll

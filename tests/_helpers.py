@@ -95,7 +95,8 @@ def cell_errors(got, refs, keys):
 def assert_parity(got, refs, band, keys, prec, label=""):
     """FP64: every cell outside the delta band within 1e-6.  Mixed / FP32 (the 1e-4 bar):
     outside the band at least 99.9 % of cells within 1e-4 per field and every cell within
-    1e-3.  Either way at most 0.5 % of all cells, band included, over the tolerance.
+    1e-3.  Either way at most 0.5 % of all cells (at least 3 allowed for small samples),
+    band included, over the tolerance.
     Knife-edge cells are checked against the branch variants, not excluded.  Prints the
     counts."""
     rel, tol = PREC[prec]
@@ -113,7 +114,7 @@ def assert_parity(got, refs, band, keys, prec, label=""):
     else:
         assert out.sum() <= max(1, int(0.001 * n)), np.where(out)[0][:20]
         assert not (failing_cells(got, refs, keys, 1e-3) & ~band).any()
-    assert bad.mean() < 0.005, bad.mean()
+    assert bad.sum() <= max(3, int(0.005 * n)), int(bad.sum())
     return bad
 
 

@@ -15,16 +15,23 @@ from ibis_amd.state import IBISState
 from ibis_amd.step import timestep_body, timestep_
 from oracle import compare, params as OP
 
-from _helpers import OUT_KEYS, assert_parity, failing_cells, oracle_with_band, workload
+from _helpers import (OUT_KEYS, assert_parity, failing_cells, oracle_with_band, site_case,
+                      water_residual, workload)
 
 pytestmark = pytest.mark.gpu
 
 DT = {"f64": torch.float64, "mixed": torch.float32, "f32": torch.float32}
 
 
-def _gpu(f, slot, prec, bedrock=True, pitch=None):
-    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda",
-                              dtype=DT[prec], pitch=pitch, bedrock=bedrock)
+def _gpu(f, slot, prec, bedrock=True, pitch=None, glob=None):
+    if glob is None:
+        st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda",
+                                  dtype=DT[prec], pitch=pitch, bedrock=bedrock)
+    else:
+        st = IBISState.from_numpy(f, slot, glob["hsoi"], glob["froot"], device="cuda",
+                                  dtype=DT[prec], pitch=pitch, bedrock=bedrock,
+                                  co2conc=glob["co2conc"], nytimes=glob["nytimes"],
+                                  snow_const=glob["snow"])
     new = timestep_body(st, get_constants(), default_sim_params(bedrock=bedrock), 3600.0,
                         precision=prec)
     torch.cuda.synchronize()
@@ -39,6 +46,23 @@ def test_step_matches_oracle(prec, hour):
     band, refs, keys, tol = oracle_with_band(f, slot, prec)
     got = _gpu(f, slot, prec)
     assert_parity(got, refs, band, keys, prec, f"gpu hour {hour}")
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("day", [None, 13])
+def test_site_state_matches_oracle(prec, day):
+    # the reference's saved 171-cell site state (a real IBIS state), night and day forcing
+    f, slot, glob = site_case(day)
+    if prec != "f64":
+        f = {k: np.asarray(v, np.float32).astype(np.float64) for k, v in f.items()}
+    band, refs, keys, tol = oracle_with_band(f, slot, prec, glob=glob)
+    got = _gpu(f, slot, prec, glob=glob)
+    assert_parity(got, refs, band, keys, prec, f"gpu site {'night' if day is None else 'day'}")
+    if prec == "f64":
+        free = f["snow.fi"] == 0.0
+        resid, scale = water_residual(f, got, glob["hsoi"], 3600.0)
+        resid = resid - got["hydrology.gadjust"] * 3600.0
+        assert np.abs(resid[free]).max() < 1e-8
 
 
 def test_step_without_bedrock_matches_oracle():
