@@ -11,9 +11,12 @@ launches the step.
 
 Reported beside the throughput:
 
-* ``roofline``: the step kernel's achieved algorithmic bandwidth -- 1,448 B per cell-hour
-  (179 fp32 words read, 180 written, 12 int8 slot ids) x cells / mean kernel duration
-  measured with HIP events on the launch stream -- against MI355X's 8 TB/s;
+* ``roofline``: the step's achieved algorithmic bandwidth -- 1,448 B per cell-hour (179 fp32
+  words read, 180 written, 12 int8 slot ids) x cells / mean duration of one ``ibis_step``
+  call (its surface, canopy and ground kernels) measured with HIP events on the launch
+  stream -- against MI355X's 8 TB/s.  ``traffic`` is the HBM bytes one call moves, from the
+  calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed in ``profiles/`` for this
+  workload (null when no summary matches the run);
 * ``cpu_baseline``: the oracle (NumPy FP64 restatement of the reference step, one core) on a
   bounded sample of the same workload on this host.
 """
@@ -31,6 +34,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+PROFILE = os.path.join(ROOT, "profiles", "step_{precision}_{cells}.md")
+
+
+def profiled_traffic(precision, cells):
+    """HBM bytes per step from the committed counter summary of this workload, or None."""
+    path = PROFILE.format(precision=precision, cells=cells)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_step:"):
+                return int(line.split(":", 1)[1].strip())
+    return None
 
 
 def main():
@@ -136,8 +152,10 @@ def main():
         "kernel_ms_mean": kmean * 1e3, "kernel_ms_min": float(np.min(kms)),
         "finite_fraction": finite,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_cell_hour": per_cell},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": profiled_traffic(args.precision, n),
+                     "bytes_per_cell_hour": per_cell,
+                     "scope": "one ibis_step call: surface + canopy + ground kernels"},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, args.cpu_hours, args.seed)
@@ -163,7 +181,19 @@ def cpu_baseline(n, hours, seed):
             if not k.startswith("_"):
                 st[k] = v
     return {"value": n * hours / t, "unit": "cell-hours/s", "cores": 1, "kind": "port",
-            "sample": f"{n:,} cells x {hours} h, NumPy FP64 oracle, one thread"}
+            "sample": f"{n:,} cells x {hours} h, NumPy FP64 oracle, one thread",
+            "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
