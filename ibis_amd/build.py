@@ -2,7 +2,9 @@
 
 ``python -m ibis_amd.build`` compiles:
 
-* ``ibis_amd/libibis_step.so``       -- the product: HIP kernels + C ABI, ``--offload-arch=gfx950``;
+* ``ibis_amd/libibis_step.so``       -- the product: HIP kernels + C ABI (the step,
+  ``include/ibis_step.h``, and the hourly driver, ``include/ibis_driver.h``),
+  ``--offload-arch=gfx950``;
 * ``ibis_amd/libibis_hostcheck.so``  -- the same step source built for the host with g++,
   loaded only by tests to check the kernel source against the oracle without a GPU.
 
@@ -48,7 +50,8 @@ def build(verbose: bool = False):
     check_layout_header()
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     out = os.path.join(HERE, "libibis_step.so")
-    msg = _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "ibis_step.hip"), "-o", out + ".tmp"])
+    msg = _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "ibis_step.hip"),
+                os.path.join(CSRC, "ibis_driver.hip"), "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     hout = os.path.join(HERE, "libibis_hostcheck.so")
     msg2 = _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "ibis_hostcheck.cpp"), "-o", hout + ".tmp"])

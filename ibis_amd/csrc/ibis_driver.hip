@@ -1,0 +1,216 @@
+// HIP kernels and C ABI of the hourly driver (include/ibis_driver.h): forcing written into
+// the state before each hour and the per-step accumulators added after it.
+//
+// One thread per cell, 256 threads per block; every plane is read and written with unit
+// stride across the wavefront.  Forcing is computed in FP64 and stored in the state's type.
+// The accumulators add in the state's type with a Kahan compensation plane per sum.
+#include <hip/hip_runtime.h>
+
+#include "ibis_driver_cell.hpp"
+#include "ibis_error.hpp"
+
+namespace ibis {
+
+constexpr int kDriverBlock = 256;
+
+template <class S>
+__device__ IBIS_INLINE void put_forcing(S* a, int64_t pitch, int64_t i, const Forcing& f) {
+  a[(int64_t)IBIS_P_ATM_TA * pitch + i] = S(f.ta);
+  a[(int64_t)IBIS_P_ATM_QA * pitch + i] = S(f.qa);
+  a[(int64_t)IBIS_P_ATM_UA * pitch + i] = S(f.ua);
+  a[(int64_t)IBIS_P_ATM_COSZEN * pitch + i] = S(f.coszen);
+  for (int b = 0; b < 2; ++b) {
+    a[(int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i] = S(f.solad[b]);
+    a[(int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i] = S(f.solai[b]);
+  }
+  a[(int64_t)IBIS_P_ATM_FIRA * pitch + i] = S(f.fira);
+  a[(int64_t)IBIS_P_ATM_RAINA * pitch + i] = S(f.raina);
+  a[(int64_t)IBIS_P_ATM_SNOWA * pitch + i] = S(f.snowa);
+}
+
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void forcing_synthetic_kernel(
+    S* a, int64_t pitch, int64_t ncell, const double* lat, uint64_t seed, int64_t cell0,
+    int jday0, int64_t hour) {
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  if (i >= ncell) return;
+  Forcing f;
+  const double ps = (double)a[(int64_t)IBIS_P_ATM_PSURF * pitch + i];
+  synthetic_forcing(f, lat[i], ps, seed, (uint64_t)(cell0 + i), jday0, hour);
+  put_forcing(a, pitch, i, f);
+}
+
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void forcing_diurnal_kernel(
+    S* a, int64_t pitch, const S* daily, int64_t dpitch, int64_t ncell, const double* lat,
+    double time, double jday, double dtime, uint64_t seed, int64_t cell0, int64_t step) {
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  if (i >= ncell) return;
+  double dw[IBIS_NDAILY];
+#pragma unroll
+  for (int j = 0; j < IBIS_NDAILY; ++j) dw[j] = (double)daily[(int64_t)j * dpitch + i];
+  const double rn = uniform01(seed, kStreamDiurnalWind, (uint64_t)(cell0 + i), (uint64_t)step);
+  Forcing f;
+  double ps;
+  diurnal_forcing(f, ps, dw, lat[i], time, jday, dtime, rn);
+  put_forcing(a, pitch, i, f);
+  a[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = S(ps);
+}
+
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void accumulate_kernel(
+    const S* st, int64_t pitch, const S* vx, int64_t vpitch, S* acc, int64_t apitch,
+    int64_t ncell, IbisAccParams p) {
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  if (i >= ncell) return;
+  auto ld = [&](int plane) { return st[(int64_t)plane * pitch + i]; };
+  AccIn<S> a;
+  for (int g = 0; g < 2; ++g) {
+    a.an_u[g] = ld(IBIS_P_CANOPY_AN_UPPER + g);
+    a.ag_u[g] = ld(IBIS_P_CANOPY_AG_UPPER + g);
+  }
+  for (int g = 0; g < 3; ++g) {
+    a.an_l[g] = ld(IBIS_P_CANOPY_AN_LOWER + g);
+    a.ag_l[g] = ld(IBIS_P_CANOPY_AG_LOWER + g);
+  }
+  a.lai_l = ld(IBIS_P_VEGETATION_LAI);
+  a.lai_u = ld(IBIS_P_VEGETATION_LAI + 1);
+  a.fu = ld(IBIS_P_VEGETATION_FU);
+  a.fl = ld(IBIS_P_VEGETATION_FL);
+  a.fi = ld(IBIS_P_SNOW_FI);
+  a.ts = ld(IBIS_P_CANOPY_TS);
+  for (int k = 0; k < IBIS_NSOILAY; ++k) a.tsoi[k] = ld(IBIS_P_SOIL_TSOI + k);
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    a.frac[j] = ld(IBIS_P_VEGETATION_FRAC + j);
+    a.cbiow[j] = vx[(int64_t)(IBIS_VEGX_CBIOW + j) * vpitch + i];
+    a.cbior[j] = vx[(int64_t)(IBIS_VEGX_CBIOR + j) * vpitch + i];
+  }
+  a.sapfrac = vx[(int64_t)IBIS_VEGX_SAPFRAC * vpitch + i];
+  a.tco2mic = vx[(int64_t)IBIS_VEGX_TCO2MIC * vpitch + i];
+  a.raina = ld(IBIS_P_ATM_RAINA);
+  a.snowa = ld(IBIS_P_ATM_SNOWA);
+  a.coszen = ld(IBIS_P_ATM_COSZEN);
+  a.ginvap = ld(IBIS_P_HYDROLOGY_GINVAP);
+  a.gsuvap = ld(IBIS_P_HYDROLOGY_GSUVAP);
+  a.gtrans = ld(IBIS_P_HYDROLOGY_GTRANS);
+  a.grunof = ld(IBIS_P_HYDROLOGY_GRUNOF);
+  a.gdrain = ld(IBIS_P_HYDROLOGY_GDRAIN);
+  a.fvapi = ld(IBIS_P_CANOPY_FVAPI);
+  S inc[IBIS_NACC];
+  acc_increments(inc, a, p);
+#pragma unroll
+  for (int j = 0; j < IBIS_NACC; ++j) {
+    S* sp = acc + (int64_t)j * apitch + i;
+    S* cp = acc + (int64_t)(IBIS_NACC + j) * apitch + i;
+    S s = *sp, c = *cp;
+    kahan_add(s, c, inc[j]);
+    *sp = s;
+    *cp = c;
+  }
+}
+
+static int launch_error(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(err_buf(), 512, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return IBIS_E_LAUNCH;
+  }
+  return IBIS_OK;
+}
+
+static int check_arena(const IbisArena* a, int64_t ncell, const char* what) {
+  if (!a || !a->planes) return fail(IBIS_E_ARG, what);
+  if (a->pitch < ncell) return fail(IBIS_E_ARG, what);
+  return IBIS_OK;
+}
+
+}  // namespace ibis
+
+extern "C" {
+
+int ibis_forcing_synthetic(const IbisArena* state, const double* lat_deg, int64_t ncell,
+                           uint64_t seed, int64_t cell0, int jday0, int64_t hour, int flags,
+                           void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0 || hour < 0 || cell0 < 0)
+    return fail(IBIS_E_ARG, "ibis_forcing_synthetic: negative size, cell or hour");
+  if (check_arena(state, ncell, "ibis_forcing_synthetic: null state or pitch < ncell"))
+    return IBIS_E_ARG;
+  if (!lat_deg) return fail(IBIS_E_ARG, "ibis_forcing_synthetic: null latitude array");
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_forcing_synthetic: unknown flag bits or precision");
+  if (ncell == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(forcing_synthetic_kernel<double>, grid, block, 0, s,
+                       (double*)state->planes, state->pitch, ncell, lat_deg, seed, cell0, jday0,
+                       hour);
+  else
+    hipLaunchKernelGGL(forcing_synthetic_kernel<float>, grid, block, 0, s,
+                       (float*)state->planes, state->pitch, ncell, lat_deg, seed, cell0, jday0,
+                       hour);
+  return launch_error("ibis_forcing_synthetic");
+}
+
+int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
+                         const double* lat_deg, int64_t ncell, double time, int jday,
+                         double dtime, uint64_t seed, int64_t cell0, int64_t step, int flags,
+                         void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0 || cell0 < 0 || step < 0 || !(dtime > 0.0))
+    return fail(IBIS_E_ARG, "ibis_forcing_diurnal: negative size, cell or step, or dtime <= 0");
+  if (check_arena(state, ncell, "ibis_forcing_diurnal: null state or pitch < ncell") ||
+      check_arena(daily, ncell, "ibis_forcing_diurnal: null daily weather or pitch < ncell"))
+    return IBIS_E_ARG;
+  if (!lat_deg) return fail(IBIS_E_ARG, "ibis_forcing_diurnal: null latitude array");
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_forcing_diurnal: unknown flag bits or precision");
+  if (ncell == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(forcing_diurnal_kernel<double>, grid, block, 0, s,
+                       (double*)state->planes, state->pitch, (const double*)daily->planes,
+                       daily->pitch, ncell, lat_deg, time, (double)jday, dtime, seed, cell0,
+                       step);
+  else
+    hipLaunchKernelGGL(forcing_diurnal_kernel<float>, grid, block, 0, s, (float*)state->planes,
+                       state->pitch, (const float*)daily->planes, daily->pitch, ncell, lat_deg,
+                       time, (double)jday, dtime, seed, cell0, step);
+  return launch_error("ibis_forcing_diurnal");
+}
+
+int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisArena* acc,
+                    const IbisAccParams* p, int64_t ncell, int flags, void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0 || !p) return fail(IBIS_E_ARG, "ibis_accumulate: negative size or null params");
+  if (check_arena(state, ncell, "ibis_accumulate: null state or pitch < ncell") ||
+      check_arena(vegx, ncell, "ibis_accumulate: null vegetation extras or pitch < ncell") ||
+      check_arena(acc, ncell, "ibis_accumulate: null accumulators or pitch < ncell"))
+    return IBIS_E_ARG;
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_accumulate: unknown flag bits or precision");
+  if (!(p->dtime > 0.0) || !(p->ndaypy > 0.0))
+    return fail(IBIS_E_ARG, "ibis_accumulate: dtime and ndaypy must be positive");
+  if (ncell == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(accumulate_kernel<double>, grid, block, 0, s,
+                       (const double*)state->planes, state->pitch, (const double*)vegx->planes,
+                       vegx->pitch, (double*)acc->planes, acc->pitch, ncell, *p);
+  else
+    hipLaunchKernelGGL(accumulate_kernel<float>, grid, block, 0, s, (const float*)state->planes,
+                       state->pitch, (const float*)vegx->planes, vegx->pitch,
+                       (float*)acc->planes, acc->pitch, ncell, *p);
+  return launch_error("ibis_accumulate");
+}
+
+}  // extern "C"

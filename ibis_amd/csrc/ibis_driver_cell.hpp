@@ -1,0 +1,276 @@
+// Per-cell bodies of the hourly driver (include/ibis_driver.h): forcing written before the
+// step and accumulators added after it.  Host-and-device code, so the host check build and
+// the HIP kernels run the same source.
+#pragma once
+
+#include <stdint.h>
+
+#include "ibis_driver.h"
+#include "ibis_math.hpp"
+
+namespace ibis {
+
+// ---- counter-based random numbers ------------------------------------------------------
+// SplitMix64's finaliser applied twice to a linear key of (seed, stream, cell, hour); the
+// top 53 bits give a double in the open interval (0, 1).  Any draw is independent of every
+// other, so cells and hours can be generated in any order and in any batch.
+IBIS_HD IBIS_INLINE uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+IBIS_HD IBIS_INLINE double uniform01(uint64_t seed, uint64_t stream, uint64_t cell,
+                                     uint64_t hour) {
+  const uint64_t x = seed * 0x9E3779B97F4A7C15ull + stream * 0xD1B54A32D192ED03ull +
+                     cell * 0xAEF17502108EF2D9ull + hour * 0xB5AD4ECEDA1CE2A9ull;
+  const uint64_t r = mix64(mix64(x));
+  return (double)(r >> 11) * 0x1.0p-53 + 0x1.0p-54;
+}
+
+// draw streams of the benchmark forcing (per cell-day: hour argument = day index)
+constexpr uint64_t kStreamTmean = 20, kStreamRh = 21, kStreamCloud = 22;
+constexpr uint64_t kStreamWind1 = 23, kStreamWind2 = 24, kStreamPwet = 25, kStreamPamt = 26;
+constexpr uint64_t kStreamDiurnalWind = 27;
+
+constexpr double kPi7 = 3.1415927;            // 7-digit pi of the benchmark generator
+constexpr double kPi = 3.141592653589793;      // weather.py's _PI (jnp.pi)
+
+// Hartmann solar declination of day `jday` (python-port/ibis/physics/weather.py:519-527)
+IBIS_HD IBIS_INLINE double declination(double jday, double pi) {
+  const double orbit = 2.0 * pi * jday / 365.2425;
+  return 0.006918 - 0.399912 * cos(orbit) + 0.070257 * sin(orbit) -
+         0.006758 * cos(2.0 * orbit) + 0.000907 * sin(2.0 * orbit) -
+         0.002697 * cos(3.0 * orbit) + 0.001480 * sin(3.0 * orbit);
+}
+
+// solar constant times the orbital factor (weather.py:448-455)
+IBIS_HD IBIS_INLINE double solar_constant(double jday, double pi) {
+  const double orbit = 2.0 * pi * jday / 365.2425;
+  return 1370.0 * (1.000110 + 0.034221 * cos(orbit) + 0.001280 * sin(orbit) +
+                   0.000719 * cos(2.0 * orbit) + 0.000077 * sin(2.0 * orbit));
+}
+
+// Spitters et al. (1986) hourly diffuse fraction from the clearness index (weather.py:458-480)
+IBIS_HD IBIS_INLINE double spitters_fd(double kt, double coszen) {
+  const double r = 0.847 - 1.61 * coszen + 1.04 * coszen * coszen;
+  const double kk = (1.47 - r) / 1.66;
+  double fd;
+  if (kt <= 0.22) fd = 1.0;
+  else if (kt <= 0.35) fd = 1.0 - 6.4 * ((kt - 0.22) * (kt - 0.22));
+  else if (kt <= kk) fd = 1.47 - 1.66 * kt;
+  else fd = r;
+  fd = fd < 0.0 ? 0.0 : (fd > 1.0 ? 1.0 : fd);
+  return coszen <= 0.0 ? 1.0 : fd;
+}
+
+// Lowe (1977) saturation vapour pressure over water / ice, Pa, and saturation humidity, in
+// the operation order of the host generator (ibis_amd/synth.py)
+IBIS_HD IBIS_INLINE double esat_d(double t) {
+  const double tm = 273.16;
+  double tl = t - tm, ti = t - tm;
+  tl = tl < 0.0 ? 0.0 : (tl > 100.0 ? 100.0 : tl);
+  ti = ti < -60.0 ? -60.0 : (ti > 0.0 ? 0.0 : ti);
+  const double a[7] = {6.1078000, 4.4365185e-1, 1.4289458e-2, 2.6506485e-4, 3.0312404e-6,
+                       2.0340809e-8, 6.1368209e-11};
+  const double b[7] = {6.1091780, 5.0346990e-1, 1.8860134e-2, 4.1762237e-4, 5.8247203e-6,
+                       4.8388032e-8, 1.8388269e-10};
+  double pa = a[6], pb = b[6];
+  for (int j = 5; j >= 1; --j) {
+    pa = a[j] + tl * pa;
+    pb = b[j] + ti * pb;
+  }
+  return 100.0 * ((t >= tm ? a[0] : b[0]) + tl * pa + ti * pb);
+}
+
+IBIS_HD IBIS_INLINE double qsat_d(double e, double p) {
+  const double d = p - 0.378 * e, f = 0.622 * e;
+  return 0.622 * e / (d > f ? d : f);
+}
+
+// Atmospheric forcing of one cell-hour (ibis_forcing_synthetic)
+struct Forcing {
+  double ta, qa, ua, coszen, solad[2], solai[2], fira, raina, snowa;
+};
+
+IBIS_HD IBIS_INLINE void synthetic_forcing(Forcing& o, double lat_deg, double psurf,
+                                           uint64_t seed, uint64_t cell, int jday0,
+                                           int64_t hour) {
+  const int64_t day = hour / 24, h = hour % 24;
+  const double jday = (double)(jday0 + day);
+  const double decl = declination(jday, kPi7);
+  const double angle = 2.0 * kPi7 * ((double)h - 12.0) / 24.0;
+  const double xlat = lat_deg * kPi7 / 180.0;
+  double cz = sin(xlat) * sin(decl) + cos(xlat) * cos(decl) * cos(angle);
+  o.coszen = cz > 0.0 ? cz : 0.0;
+  const double cf = 0.3 + 0.7 * uniform01(seed, kStreamCloud, cell, (uint64_t)day);
+  const double sw = 1000.0 * o.coszen * cf;
+  const double s0 = solar_constant(jday, kPi7);
+  double den = s0 * o.coszen;
+  den = den > 1.0 ? den : 1.0;
+  double kt = sw / den;
+  kt = kt < 0.0 ? 0.0 : (kt > 1.0 ? 1.0 : kt);
+  const double fd = spitters_fd(kt, o.coszen);
+  const double band[2] = {0.46, 0.54};
+  for (int b = 0; b < 2; ++b) {
+    o.solad[b] = sw * band[b] * (1.0 - fd);
+    o.solai[b] = sw * band[b] * fd;
+  }
+  const double tmean = 250.0 + 60.0 * uniform01(seed, kStreamTmean, cell, (uint64_t)day);
+  const double pi = 3.141592653589793;
+  o.ta = tmean + 8.0 * cos(2.0 * pi * ((double)h - 15.0) / 24.0);
+  const double rh = 0.2 + 0.8 * uniform01(seed, kStreamRh, cell, (uint64_t)day);
+  o.qa = rh * qsat_d(esat_d(o.ta), psurf);
+  const double wind_mu = log(3.0) - 0.5 * 0.5 * 0.5;
+  const double z1 = uniform01(seed, kStreamWind1, cell, (uint64_t)hour);
+  const double z2 = uniform01(seed, kStreamWind2, cell, (uint64_t)hour);
+  const double gauss = sqrt(-2.0 * log(z1)) * cos(2.0 * pi * z2);
+  o.ua = exp(wind_mu + 0.5 * gauss);
+  const bool wet = uniform01(seed, kStreamPwet, cell, (uint64_t)hour) < 0.1;
+  const double amount = -2.0 * log(uniform01(seed, kStreamPamt, cell, (uint64_t)hour));
+  const double rate = wet ? amount / 3600.0 : 0.0;
+  const bool warm = (o.ta - 273.15) > 2.5;
+  o.raina = warm ? rate : 0.0;
+  o.snowa = warm ? 0.0 : rate;
+  const double cloud = 1.0 - cf;
+  const double emb = 0.01 * (psurf * o.qa / (0.622 + o.qa));
+  const double ea = 0.70 + 5.95e-5 * emb * exp(1500.0 / o.ta);
+  const double t4 = (o.ta * o.ta) * (o.ta * o.ta);
+  const double stef = 5.67e-8;
+  o.fira = (1.0 - cloud) * ea * stef * t4 + cloud * 0.950 * stef * t4;
+}
+
+// The reference's diurnal disaggregation (weather.py:483-601) for one cell, given the day's
+// weather and precipitation event and the step's uniform wind draw.
+IBIS_HD IBIS_INLINE void diurnal_forcing(Forcing& o, double& psurf_out,
+                                         const double dw[IBIS_NDAILY], double lat_deg,
+                                         double time, double jday, double dtime,
+                                         double rn_wind) {
+  const double td = dw[0], tmax = dw[1], tmin = dw[2], qd = dw[3], ud = dw[4];
+  const double precip = dw[5], cloud = dw[6], psurf = dw[7];
+  const double plens = dw[8], startp = dw[9], endp = dw[10];
+  (void)td;
+  const double pi = kPi;
+  const double rtime = time / 3600.0;
+  const double xdecl = declination(jday, pi);
+  const double angle = 2.0 * pi * (rtime - 12.0) / 24.0;
+  const double sw = solar_constant(jday, pi);
+  const double xlat = lat_deg * pi / 180.0;
+  const double cz = sin(xlat) * sin(xdecl) + cos(xlat) * cos(xdecl) * cos(angle);
+  o.coszen = cz > 0.0 ? cz : 0.0;
+  const double trans = 0.251 + 0.509 * (1.0 - cloud);
+  const double fdiff = trans > 0.75 ? 0.166
+                                    : 1.0045 + 0.0435 * trans - 3.5227 * (trans * trans) +
+                                          2.6313 * (trans * (trans * trans));
+  const double frac[2] = {0.46, 0.54};
+  for (int b = 0; b < 2; ++b) {
+    o.solad[b] = sw * o.coszen * frac[b] * trans * (1.0 - fdiff);
+    o.solai[b] = sw * o.coszen * frac[b] * trans * fdiff;
+  }
+  const double gamma = 0.44 - 0.46 * sin(pi / 12.0 * rtime + 0.9) +
+                       0.11 * sin(2.0 * pi / 12.0 * rtime + 0.9);
+  o.ta = tmax * gamma + tmin * (1.0 - gamma);
+  double qmin = 0.99 * qsat(esat(tmin), psurf);
+  qmin = qd < qmin ? qd : qmin;
+  const double qmax = (qd - 0.56 * qmin) / 0.44;
+  const double qsa = 0.99 * qsat(esat(o.ta), psurf);
+  const double qc = qmax * gamma + qmin * (1.0 - gamma);
+  o.qa = qsa < qc ? qsa : qc;
+  double u = 1.13989 * ud * pow(-log(rn_wind > 1e-10 ? rn_wind : 1e-10), 0.30);
+  o.ua = u < 2.5 ? 2.5 : (u > 10.0 ? 10.0 : u);
+  const double emb = 0.01 * (psurf * o.qa / (0.622 + o.qa));
+  const double ea = 0.70 + 5.95e-5 * emb * exp(1500.0 / o.ta);
+  const double t4 = pow4(o.ta);
+  const double stef = 5.67e-8;
+  o.fira = (1.0 - cloud) * ea * stef * t4 + cloud * 0.950 * stef * t4;
+  const bool in_event = (time >= startp) && (time < endp);
+  const bool warm = (o.ta - 273.15) > 2.5;
+  const double rate = precip / (plens > dtime ? plens : dtime);
+  o.raina = (in_event && warm) ? rate : 0.0;
+  o.snowa = (in_event && !warm) ? rate : 0.0;
+  psurf_out = psurf;
+}
+
+// ---- per-step accumulators (loops_gswp3.py:161-302) ------------------------------------
+
+// Kahan-Babuska compensated add: sum + comp carries the running total to twice the
+// working precision; `comp` holds the low-order part lost by the last addition.
+template <class T> IBIS_HD IBIS_INLINE void kahan_add(T& sum, T& comp, T x) {
+  const T y = x - comp;
+  const T t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+// What the accumulators read of the state after the step, per cell.
+template <class R> struct AccIn {
+  R an_u[2], an_l[3], ag_u[2], ag_l[3];
+  R lai_l, lai_u, fu, fl, fi, ts, tsoi[IBIS_NSOILAY];
+  R frac[IBIS_NPFT], cbiow[IBIS_NPFT], cbior[IBIS_NPFT], sapfrac, tco2mic;
+  R raina, snowa, coszen, ginvap, gsuvap, gtrans, grunof, gdrain, fvapi;
+};
+
+// The step's increment of each accumulator (same order as IbisAcc).
+template <class R>
+IBIS_HD IBIS_INLINE void acc_increments(R inc[IBIS_NACC], const AccIn<R>& a,
+                                        const IbisAccParams& p) {
+  const R dt = R(p.dtime);
+  const R smask = R(1) - a.fi;
+  R roottemp = R(0);
+#pragma unroll
+  for (int k = 0; k < IBIS_NSOILAY; ++k)
+    roottemp = roottemp + a.tsoi[k] * R(0.5) * (R(p.froot[k][0]) + R(p.froot[k][1]));
+  const R funca = r_exp(R(3500) * (R(1.0 / 288.16) - R(1) / mx(a.ts, R(200))));
+  const R funcb = r_exp(R(3500) * (R(1.0 / 288.16) - R(1) / mx(roottemp, R(200))));
+  const R c_fac = R(12.0e-3) * dt;
+  const R per_s = R(1.0 / (p.ndaypy * 86400.0) * (1000.0 / 12.0));
+  // big leaf feeding each PFT: upper broadleaf 0 1 2 4 6, upper conifer 3 5 7, shrubs 8 9,
+  // C4 grass 10, C3 grass 11
+  R npp_step = R(0);
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    const bool conifer = (j == 3 || j == 5 || j == 7);
+    const bool upper = j < 8;
+    R an;
+    if (upper) an = conifer ? a.an_u[1] : a.an_u[0];
+    else if (j < 10) an = a.an_l[0];
+    else an = (j == 10) ? a.an_l[2] : a.an_l[1];
+    const R gpp = upper ? an * a.frac[j] * a.lai_u * a.fu
+                        : an * a.frac[j] * a.lai_l * a.fl * smask;
+    const R rw = R(p.rwood[j]) * per_s, rr = R(p.rroot[j]) * per_s;
+    R resp;
+    if (upper) resp = rw * a.cbiow[j] * a.sapfrac * funca + rr * a.cbior[j] * funcb;
+    else if (j < 10) resp = rw * a.cbiow[j] * funca + rr * a.cbior[j] * funcb;
+    else resp = rr * a.cbior[j] * funcb;
+    const R npp = (R(1) - R(p.rgrowth[j])) * (gpp - resp);
+    inc[IBIS_ACC_AYNPP + j] = npp * c_fac;
+    npp_step = npp_step + npp * c_fac;
+  }
+  const R fb = a.frac[0] + a.frac[1] + a.frac[2] + a.frac[4] + a.frac[6];
+  const R fc = a.frac[3] + a.frac[5] + a.frac[7];
+  const R gpp_u = (a.ag_u[0] * fb + a.ag_u[1] * fc) * a.lai_u * a.fu;
+  const R gpp_l = (a.ag_l[0] * (a.frac[8] + a.frac[9]) + a.ag_l[2] * a.frac[10] +
+                   a.ag_l[1] * a.frac[11]) * a.lai_l * a.fl * smask;
+  inc[IBIS_ACC_ADRAIN] = a.raina * dt;
+  inc[IBIS_ACC_ADSNOW] = a.snowa * dt;
+  inc[IBIS_ACC_ADAET] = (a.ginvap + a.gsuvap + a.gtrans) * dt;
+  inc[IBIS_ACC_A10ANCUB] = a.an_u[0] * dt;
+  inc[IBIS_ACC_A10ANCLS] = a.an_l[0] * dt;
+  inc[IBIS_ACC_A10ANCL3] = a.an_l[1] * dt;
+  inc[IBIS_ACC_A10ANCL4] = a.an_l[2] * dt;
+  inc[IBIS_ACC_AYNPPTOT] = npp_step;
+  inc[IBIS_ACC_AYGPPTOT] = (gpp_u + gpp_l) * c_fac;
+  inc[IBIS_ACC_AYTRUNOF] = a.grunof * dt;
+  inc[IBIS_ACC_AYTDRAIN] = a.gdrain * dt;
+  const R snowsub = a.fvapi * a.fi * dt;
+  inc[IBIS_ACC_ET_TRANS] = a.gtrans * dt;
+  inc[IBIS_ACC_ET_INTERC] = a.ginvap * dt;
+  inc[IBIS_ACC_ET_SOILEV] = a.gsuvap * dt - snowsub;
+  inc[IBIS_ACC_ET_SNOWSUB] = snowsub;
+  const R nee = a.tco2mic * c_fac - npp_step;
+  const bool night = a.coszen <= R(0);
+  inc[IBIS_ACC_NEE_NIGHT] = night ? nee : R(0);
+  inc[IBIS_ACC_NEE_DAY] = night ? R(0) : nee;
+}
+
+}  // namespace ibis

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "ibis_error.hpp"
 #include "ibis_step_cell.hpp"
 
 namespace ibis {
@@ -78,13 +79,6 @@ __global__ __launch_bounds__(kBlock, kWavesGround) void ground_kernel(
   ground_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
 }
 
-static thread_local char g_err[512] = "";
-
-static int fail(int code, const char* msg) {
-  snprintf(g_err, sizeof(g_err), "%s", msg);
-  return code;
-}
-
 template <class S, class R, class D>
 static int launch(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
                   int64_t slot_pitch, const double* table, const IbisParams& P,
@@ -101,7 +95,7 @@ static int launch(const IbisArena* in, const IbisArena* out, void* ws, const int
                      P, K);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    snprintf(g_err, sizeof(g_err), "ibis_step: launch failed: %s", hipGetErrorString(e));
+    snprintf(err_buf(), 512, "ibis_step: launch failed: %s", hipGetErrorString(e));
     return IBIS_E_LAUNCH;
   }
   return IBIS_OK;
@@ -119,7 +113,7 @@ static int64_t workspace_bytes(int64_t pitch, int prec) {
 
 extern "C" {
 
-const char* ibis_last_error(void) { return ibis::g_err; }
+const char* ibis_last_error(void) { return ibis::err_buf(); }
 
 const char* ibis_version(void) { return "ibis_amd 0.2 (gfx950, three-phase step)"; }
 
@@ -134,7 +128,7 @@ int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
               const double* type_table, const IbisParams* params, double dtime, double nytimes,
               int64_t ncell, int flags, void* hip_stream) {
   using namespace ibis;
-  g_err[0] = 0;
+  err_buf()[0] = 0;
   if (!in || !out || !params || !type_table || !in->planes || !out->planes)
     return fail(IBIS_E_ARG, "ibis_step: null pointer argument");
   if (ncell < 0) return fail(IBIS_E_ARG, "ibis_step: negative ncell");

@@ -1,0 +1,207 @@
+"""The hourly driver around the step, on the GPU: forcing written before each hour, the step,
+and the per-step accumulators after it.
+
+Mirrors the body of the reference's hourly scan (``python-port/ibis/timestepping/loops_gswp3.py:115-302``):
+the forcing write of ``:142-152``, ``timestep_body``, and the accumulators of ``:161-302``.
+Every hour is three C-ABI calls on one stream (``include/ibis_driver.h``,
+``include/ibis_step.h``), with no host synchronisation inside a run.
+
+* ``SyntheticForcing`` -- the benchmark forcing (``ibis_forcing_synthetic``), the device
+  twin of ``ibis_amd.synth.forcing``.
+* ``DiurnalForcing`` -- the reference's daily-to-hourly disaggregation
+  (``ibis_forcing_diurnal``; ``python-port/ibis/physics/weather.py:483-601``) from per-cell
+  daily weather.
+* ``Accumulators`` -- the reference's per-step sums (``ibis_accumulate``), each with a Kahan
+  compensation plane, so the FP32 production build keeps FP64-grade sums over a year.
+* ``run_hours`` -- the loop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import abi
+from . import layout as L
+from . import step as S
+from .state import IBISState
+
+#: accumulator names, in the order of include/ibis_driver.h's IbisAcc
+ACC_NAMES = (["adrain", "adsnow", "adaet", "a10ancub", "a10ancls", "a10ancl3", "a10ancl4"]
+             + [f"aynpp{j}" for j in range(L.NPFT)]
+             + ["aynpptot", "aygpptot", "aytrunof", "aytdrain", "et_trans", "et_interc",
+                "et_soilev", "et_snowsub", "nee_night", "nee_day"])
+NACC = len(ACC_NAMES)
+NVEGX = 26
+DAILY_NAMES = ("td", "tmax", "tmin", "qd", "ud", "precip", "cloud", "psurf", "plens",
+               "startp", "endp")
+
+
+class IbisAccParams(C.Structure):
+    _fields_ = [("rwood", C.c_double * L.NPFT), ("rroot", C.c_double * L.NPFT),
+                ("rgrowth", C.c_double * L.NPFT), ("froot", (C.c_double * 2) * L.NSOILAY),
+                ("dtime", C.c_double), ("ndaypy", C.c_double)]
+
+
+@dataclass
+class RespirationParams:
+    """Maintenance and growth respiration of the NPP accumulator
+    (``python-port/ibis/io/parameters.py:248-259``): fractions per year at 15 C."""
+    rwood: np.ndarray = field(default_factory=lambda: np.full(L.NPFT, 0.0125))
+    rroot: np.ndarray = field(default_factory=lambda: np.full(L.NPFT, 1.25))
+    rgrowth: np.ndarray = field(default_factory=lambda: np.full(L.NPFT, 0.30))
+    ndaypy: float = 365.0
+
+
+def acc_params(state: IBISState, dtime: float, resp: RespirationParams | None = None):
+    r = resp or RespirationParams()
+    p = IbisAccParams()
+    p.rwood[:] = [float(x) for x in np.broadcast_to(r.rwood, (L.NPFT,))]
+    p.rroot[:] = [float(x) for x in np.broadcast_to(r.rroot, (L.NPFT,))]
+    p.rgrowth[:] = [float(x) for x in np.broadcast_to(r.rgrowth, (L.NPFT,))]
+    for k in range(L.NSOILAY):
+        p.froot[k][0] = float(state.froot[k, 0])
+        p.froot[k][1] = float(state.froot[k, 1])
+    p.dtime = float(dtime)
+    p.ndaypy = float(r.ndaypy)
+    return p
+
+
+def _lib_driver():
+    lib = _lib.load()
+    if not getattr(lib, "_driver_typed", False):
+        lib.ibis_forcing_synthetic.argtypes = [C.POINTER(abi.IbisArena), C.c_void_p, C.c_int64,
+                                               C.c_uint64, C.c_int64, C.c_int, C.c_int64, C.c_int,
+                                               C.c_void_p]
+        lib.ibis_forcing_synthetic.restype = C.c_int
+        lib.ibis_forcing_diurnal.argtypes = [C.POINTER(abi.IbisArena), C.POINTER(abi.IbisArena),
+                                             C.c_void_p, C.c_int64, C.c_double, C.c_int,
+                                             C.c_double, C.c_uint64, C.c_int64, C.c_int64,
+                                             C.c_int, C.c_void_p]
+        lib.ibis_forcing_diurnal.restype = C.c_int
+        lib.ibis_accumulate.argtypes = [C.POINTER(abi.IbisArena), C.POINTER(abi.IbisArena),
+                                        C.POINTER(abi.IbisArena), C.POINTER(IbisAccParams),
+                                        C.c_int64, C.c_int, C.c_void_p]
+        lib.ibis_accumulate.restype = C.c_int
+        lib._driver_typed = True
+    return lib
+
+
+def _code(state):
+    return abi.PREC_F64 if state.dtype == torch.float64 else abi.PREC_MIXED
+
+
+def _stream(state, stream):
+    return stream if stream is not None else torch.cuda.current_stream(state.device)
+
+
+class SyntheticForcing:
+    """Benchmark forcing of each hour written into the state (``ibis_forcing_synthetic``)."""
+
+    def __init__(self, lat_deg, seed: int, jday0: int = 172, cell0: int = 0, device="cuda"):
+        self.lat = torch.as_tensor(np.asarray(lat_deg, dtype=np.float64), device=device)
+        self.seed, self.jday0, self.cell0 = int(seed), int(jday0), int(cell0)
+
+    def write(self, state: IBISState, hour: int, stream=None):
+        lib = _lib_driver()
+        st = _stream(state, stream)
+        a = abi.IbisArena(state.arena.data_ptr(), state.pitch)
+        rc = lib.ibis_forcing_synthetic(C.byref(a), C.c_void_p(self.lat.data_ptr()),
+                                        C.c_int64(state.n), C.c_uint64(self.seed),
+                                        C.c_int64(self.cell0), C.c_int(self.jday0),
+                                        C.c_int64(hour), C.c_int(_code(state)),
+                                        C.c_void_p(st.cuda_stream))
+        _lib.check(rc, "ibis_forcing_synthetic")
+
+
+class DiurnalForcing:
+    """The reference's diurnal disaggregation of per-cell daily weather
+    (``ibis_forcing_diurnal``).  ``daily``: dict of the ``DAILY_NAMES`` fields (N,)."""
+
+    def __init__(self, daily: dict, lat_deg, jday: int, seed: int, dtime: float = 3600.0,
+                 cell0: int = 0, device="cuda", dtype=torch.float32):
+        n = len(np.asarray(lat_deg))
+        host = np.stack([np.asarray(daily[k], dtype=np.float64) for k in DAILY_NAMES])
+        self.daily = torch.as_tensor(host, device=device).to(dtype).contiguous()
+        self.lat = torch.as_tensor(np.asarray(lat_deg, dtype=np.float64), device=device)
+        self.n, self.jday, self.seed, self.dtime, self.cell0 = n, int(jday), int(seed), float(dtime), int(cell0)
+
+    def write(self, state: IBISState, hour: int, stream=None):
+        lib = _lib_driver()
+        st = _stream(state, stream)
+        if self.daily.dtype != state.dtype:
+            raise ValueError("daily weather and state must share a dtype")
+        a = abi.IbisArena(state.arena.data_ptr(), state.pitch)
+        d = abi.IbisArena(self.daily.data_ptr(), self.daily.shape[1])
+        time = (hour % int(round(86400.0 / self.dtime))) * self.dtime
+        rc = lib.ibis_forcing_diurnal(C.byref(a), C.byref(d), C.c_void_p(self.lat.data_ptr()),
+                                      C.c_int64(state.n), C.c_double(time), C.c_int(self.jday),
+                                      C.c_double(self.dtime), C.c_uint64(self.seed),
+                                      C.c_int64(self.cell0), C.c_int64(hour),
+                                      C.c_int(_code(state)), C.c_void_p(st.cuda_stream))
+        _lib.check(rc, "ibis_forcing_diurnal")
+
+
+class Accumulators:
+    """Per-step sums of the reference's driver with Kahan compensation.  ``vegx``: dict with
+    cbiow (N, 12), cbior (N, 12), sapfrac (N,), tco2mic (N,) (zeros when omitted)."""
+
+    def __init__(self, state: IBISState, vegx: dict | None = None, dtime: float = 3600.0,
+                 resp: RespirationParams | None = None):
+        n, pitch = state.n, state.pitch
+        host = np.zeros((NVEGX, pitch))
+        if vegx:
+            host[0:12, :n] = np.asarray(vegx["cbiow"]).T
+            host[12:24, :n] = np.asarray(vegx["cbior"]).T
+            host[24, :n] = vegx["sapfrac"]
+            host[25, :n] = vegx["tco2mic"]
+        self.vegx = torch.as_tensor(host, device=state.device).to(state.dtype).contiguous()
+        self.acc = torch.zeros((2 * NACC, pitch), dtype=state.dtype, device=state.device)
+        self.params = acc_params(state, dtime, resp)
+        self.n = n
+
+    def zero_(self, names=None):
+        if names is None:
+            self.acc.zero_()
+        else:
+            for nm in names:
+                j = ACC_NAMES.index(nm)
+                self.acc[j].zero_()
+                self.acc[NACC + j].zero_()
+
+    def add(self, state: IBISState, stream=None):
+        lib = _lib_driver()
+        st = _stream(state, stream)
+        a = abi.IbisArena(state.arena.data_ptr(), state.pitch)
+        v = abi.IbisArena(self.vegx.data_ptr(), state.pitch)
+        s = abi.IbisArena(self.acc.data_ptr(), state.pitch)
+        rc = lib.ibis_accumulate(C.byref(a), C.byref(v), C.byref(s), C.byref(self.params),
+                                 C.c_int64(state.n), C.c_int(_code(state)),
+                                 C.c_void_p(st.cuda_stream))
+        _lib.check(rc, "ibis_accumulate")
+
+    def values(self) -> dict:
+        """Compensated totals (sum minus the pending compensation), float64 numpy (N,)."""
+        tot = (self.acc[:NACC].double() - self.acc[NACC:].double())[:, :self.n].cpu().numpy()
+        return {nm: tot[j] for j, nm in enumerate(ACC_NAMES)}
+
+
+def run_hours(state: IBISState, hours: int, forcing, consts, params, dtime: float = 3600.0,
+              precision=None, acc: Accumulators | None = None, hour0: int = 0, stream=None,
+              on_hour=None):
+    """Step ``state`` in place through ``hours`` hours: forcing, step, accumulators.  All
+    launches go to one stream; nothing synchronises.  ``on_hour(h)`` runs after each hour's
+    launches (e.g. to record events)."""
+    st = _stream(state, stream)
+    cparams = S.ibis_params(state, consts, params)
+    for h in range(hour0, hour0 + hours):
+        forcing.write(state, h, stream=st)
+        S.timestep_(state, consts, params, dtime, precision=precision, stream=st, cparams=cparams)
+        if acc is not None:
+            acc.add(state, stream=st)
+        if on_hour is not None:
+            on_hour(h)
+    return state

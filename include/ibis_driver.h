@@ -1,0 +1,97 @@
+/*
+ * C ABI of the hourly driver around the step: forcing written into the state before each
+ * hour, and the per-step accumulators added after it.
+ *
+ * Replaces the body of the reference's hourly scan around timestep_body,
+ * python-port/ibis/timestepping/loops_gswp3.py:115-302: the forcing write of :142-152 and
+ * the accumulators of :161-302.  Plain pointers and sizes; all arrays are device memory;
+ * stream-ordered; nothing is allocated and nothing synchronises.  Return value as
+ * ibis_step (0, or a negative IBIS_E_* code; ibis_last_error() holds the message).
+ */
+#ifndef IBIS_DRIVER_H
+#define IBIS_DRIVER_H
+
+#include <stdint.h>
+#include "ibis_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Benchmark forcing for one hour, written into the atm planes of `state` (ta, qa, ua,
+ * coszen, solad, solai, fira, raina, snowa; psurf is read from the state): solar geometry
+ * from `lat_deg` (ncell doubles) with the Hartmann declination of day jday0 + hour / 24,
+ * shortwave 1000 W m-2 x cos(zenith) x cloud factor, Spitters diffuse split, diurnal air
+ * temperature, relative humidity, lognormal wind, Bernoulli x exponential precipitation and
+ * Idso longwave.  Random numbers come from a counter-based hash of (seed, stream, cell0 + i,
+ * hour), so any cell-hour is reproducible alone.  No counterpart in the reference: its
+ * benchmark forcing (the distributions of the benchmark configurations) is defined here; the
+ * reference's own daily-to-hourly generator is ibis_forcing_diurnal. */
+int ibis_forcing_synthetic(const IbisArena* state, const double* lat_deg, int64_t ncell,
+                           uint64_t seed, int64_t cell0, int jday0, int64_t hour, int flags,
+                           void* hip_stream);
+
+/* Daily weather of each cell (plane-major, IBIS_NDAILY planes of `pitch`, in the state's
+ * storage type): td, tmax, tmin, qd, ud, precip (mm/day), cloud, psurf, plens, startp, endp
+ * (the day's precipitation event length, start and end, s; compute_precip_timing). */
+#define IBIS_NDAILY 11
+
+/* The reference's diurnal disaggregation for the step starting `time` seconds after
+ * midnight of day `jday` (python-port/ibis/physics/weather.py:483-601): solar geometry,
+ * Friend transmissivity and diffuse fraction, Campbell-Norman temperature cycle, humidity
+ * capped at 99 % saturation, EPIC wind from a uniform draw, Idso longwave, and the day's
+ * precipitation event as rain or snow; written into the atm planes of `state` (psurf
+ * included).  The wind's uniform draw comes from the counter-based hash of (seed, cell0 + i,
+ * step) in place of the reference's jax.random key, which no other implementation can
+ * reproduce bit for bit. */
+int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
+                         const double* lat_deg, int64_t ncell, double time, int jday,
+                         double dtime, uint64_t seed, int64_t cell0, int64_t step, int flags,
+                         void* hip_stream);
+
+/* Per-step accumulators: IBIS_NACC sums, each with a compensation plane (Kahan), so that
+ * an FP32 accumulator over a year of hours keeps the accuracy of an FP64 one. */
+enum IbisAcc {
+  IBIS_ACC_ADRAIN = 0, IBIS_ACC_ADSNOW, IBIS_ACC_ADAET,
+  IBIS_ACC_A10ANCUB, IBIS_ACC_A10ANCLS, IBIS_ACC_A10ANCL3, IBIS_ACC_A10ANCL4,
+  IBIS_ACC_AYNPP,                        /* 12 planes, one per PFT */
+  IBIS_ACC_AYNPPTOT = IBIS_ACC_AYNPP + 12, IBIS_ACC_AYGPPTOT, IBIS_ACC_AYTRUNOF,
+  IBIS_ACC_AYTDRAIN,
+  IBIS_ACC_ET_TRANS, IBIS_ACC_ET_INTERC, IBIS_ACC_ET_SOILEV, IBIS_ACC_ET_SNOWSUB,
+  IBIS_ACC_NEE_NIGHT, IBIS_ACC_NEE_DAY,
+  IBIS_NACC
+};
+
+/* Vegetation leaves the accumulators read that the step does not (plane-major, state
+ * storage type): cbiow (12), cbior (12), sapfrac, tco2mic. */
+#define IBIS_VEGX_CBIOW 0
+#define IBIS_VEGX_CBIOR 12
+#define IBIS_VEGX_SAPFRAC 24
+#define IBIS_VEGX_TCO2MIC 25
+#define IBIS_NVEGX 26
+
+typedef struct IbisAccParams {
+  double rwood[IBIS_NPFT];    /* maintenance respiration of sapwood, fraction per year at 15 C */
+  double rroot[IBIS_NPFT];    /* of fine roots */
+  double rgrowth[IBIS_NPFT];  /* growth respiration, fraction of GPP - maintenance */
+  double froot[IBIS_NSOILAY][2];
+  double dtime;
+  double ndaypy;              /* days per year of the respiration rates */
+} IbisAccParams;
+
+/* Add one step to the accumulators (python-port/ibis/timestepping/loops_gswp3.py:161-302):
+ * daily rain, snow and AET; daily sums of the big leaves' net photosynthesis; NPP per PFT
+ * with Lloyd-Taylor maintenance respiration and growth respiration; total NPP and GPP;
+ * runoff and drainage; the ET components; night and day NEE.  `state` is the state after
+ * the step.  `acc` holds 2 x IBIS_NACC planes: the sums, then their compensations.  The
+ * step's NPP increment is summed from the per-PFT increments themselves rather than as the
+ * difference of the updated and old sums (loops_gswp3.py:236), which is the same quantity
+ * without the cancellation. */
+int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisArena* acc,
+                    const IbisAccParams* p, int64_t ncell, int flags, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IBIS_DRIVER_H */

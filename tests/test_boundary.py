@@ -27,8 +27,10 @@ def test_io_contract_matches_the_reference_leaf_count():
 
 def test_library_exports_every_declared_symbol():
     lib = _lib.load()
-    with open(os.path.join(ROOT, "include", "ibis_step.h")) as fh:
-        decl = re.findall(r"^(?:int|int64_t|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
+    decl = []
+    for h in ("ibis_step.h", "ibis_driver.h"):
+        with open(os.path.join(ROOT, "include", h)) as fh:
+            decl += re.findall(r"^(?:int|int64_t|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
     assert set(decl) == set(_lib.EXPORTS)
     for name in decl:
         assert hasattr(lib, name)

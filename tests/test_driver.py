@@ -1,0 +1,223 @@
+"""The hourly driver around the step (include/ibis_driver.h): oracle pins and the host build of
+the driver's per-cell source against the oracle and the host forcing generator.
+
+The GPU kernels are checked in ``test_gpu_driver.py``.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from ibis_amd import abi, driver as DR, synth
+from oracle import driver as OD, thermo
+
+from _helpers import HOSTCHECK, workload
+
+
+def _lib():
+    return C.CDLL(HOSTCHECK)
+
+
+# ---- oracle pins ------------------------------------------------------------------------
+
+def _daily(n, seed=3):
+    rng = np.random.default_rng(seed)
+    tmin = rng.uniform(250.0, 295.0, n)
+    tmax = tmin + rng.uniform(2.0, 18.0, n)
+    precip = np.where(rng.random(n) < 0.5, rng.exponential(8.0, n), 0.0)
+    plens, startp, endp = OD.precip_timing(precip, 3600.0, rng.random(n), rng.random(n))
+    psurf = rng.uniform(70000.0, 103000.0, n)
+    td = 0.5 * (tmin + tmax)
+    qd = rng.uniform(0.3, 0.9, n) * thermo.qsat(thermo.esat(td), psurf)
+    return dict(td=td, tmax=tmax, tmin=tmin, qd=qd, ud=rng.uniform(1.0, 8.0, n), precip=precip,
+                cloud=rng.uniform(0.0, 1.0, n), psurf=psurf, plens=plens, startp=startp,
+                endp=endp)
+
+
+def test_diurnal_recovers_the_days_precipitation_and_bounds():
+    n = 2000
+    dw = _daily(n)
+    lat = np.random.default_rng(4).uniform(-56.0, 84.0, n)
+    total = np.zeros(n)
+    for h in range(24):
+        out = OD.diurnal(h * 3600.0, 172.0, dw, lat, 3600.0, np.full(n, 0.37))
+        total += (out["atm.raina"] + out["atm.snowa"]) * 3600.0
+        assert np.all(out["atm.ta"] <= dw["tmax"] + 1e-9) and np.all(out["atm.ta"] >= dw["tmin"] - 1e-9)
+        qs = thermo.qsat(thermo.esat(out["atm.ta"]), dw["psurf"])
+        assert np.all(out["atm.qa"] <= 0.99 * qs * (1 + 1e-12))
+        assert np.all((out["atm.ua"] >= 2.5) & (out["atm.ua"] <= 10.0))
+    np.testing.assert_allclose(total, dw["precip"], rtol=1e-12, atol=1e-12)
+
+
+def test_diurnal_noon_zenith_is_the_closed_form():
+    lat = np.linspace(-50.0, 50.0, 11)
+    dw = _daily(11)
+    jday = 80.0
+    out = OD.diurnal(12 * 3600.0, jday, dw, lat, 3600.0, np.full(11, 0.5))
+    orbit = 2.0 * np.pi * jday / 365.2425
+    decl = (0.006918 - 0.399912 * np.cos(orbit) + 0.070257 * np.sin(orbit)
+            - 0.006758 * np.cos(2 * orbit) + 0.000907 * np.sin(2 * orbit)
+            - 0.002697 * np.cos(3 * orbit) + 0.001480 * np.sin(3 * orbit))
+    np.testing.assert_allclose(out["atm.coszen"], np.cos(np.radians(lat) - decl), rtol=1e-12)
+
+
+def test_precip_timing_keeps_rates_under_15mm_per_step():
+    rng = np.random.default_rng(5)
+    precip = rng.exponential(60.0, 5000)
+    plens, startp, endp = OD.precip_timing(precip, 3600.0, rng.random(5000), rng.random(5000))
+    assert np.all(endp <= 86400.0) and np.all(startp >= 0.0)
+    assert np.all(plens >= 4 * 3600.0)
+    heavy = precip <= 24 * 15.0
+    assert np.all(precip[heavy] / (plens[heavy] / 3600.0) <= 15.0 + 1e-9)
+
+
+def _acc_case(n, seed=7):
+    f, slot = workload(n, seed=seed, hour=13, prec="f64")
+    rng = np.random.default_rng(seed)
+    st = dict(f)
+    st["canopy.an_upper"] = rng.uniform(-1e-6, 2e-5, (n, 2))
+    st["canopy.an_lower"] = rng.uniform(-1e-6, 2e-5, (n, 3))
+    st["canopy.ag_upper"] = st["canopy.an_upper"] + 1e-6
+    st["canopy.ag_lower"] = st["canopy.an_lower"] + 1e-6
+    st["canopy.ts"] = rng.uniform(260.0, 310.0, n)
+    st["canopy.fvapi"] = rng.uniform(-1e-6, 1e-5, n)
+    for k in ("ginvap", "gsuvap", "gtrans", "grunof", "gdrain"):
+        st[f"hydrology.{k}"] = rng.uniform(0.0, 1e-4, n)
+    vx = dict(cbiow=rng.uniform(0, 10, (n, 12)), cbior=rng.uniform(0, 1, (n, 12)),
+              sapfrac=rng.uniform(0, 0.5, n), tco2mic=rng.uniform(0, 1e-6, n))
+    p = DR.RespirationParams()
+    from types import SimpleNamespace
+    from oracle import params as OP
+    op = SimpleNamespace(rwood=p.rwood, rroot=p.rroot, rgrowth=p.rgrowth,
+                         froot=OP.root_fractions(), dtime=3600.0, ndaypy=p.ndaypy)
+    return st, vx, op
+
+
+def test_accumulator_identities():
+    st, vx, p = _acc_case(3000)
+    inc = OD.acc_increments(st, vx, p)
+    ix = {k: i for i, k in enumerate(OD.ACC_NAMES)}
+    np.testing.assert_allclose(inc[7:19].sum(axis=0), inc[ix["aynpptot"]], rtol=1e-12, atol=1e-20)
+    et = inc[ix["et_trans"]] + inc[ix["et_interc"]] + inc[ix["et_soilev"]] + inc[ix["et_snowsub"]]
+    np.testing.assert_allclose(et, inc[ix["adaet"]], rtol=1e-12, atol=1e-20)
+    nee = inc[ix["nee_night"]] + inc[ix["nee_day"]]
+    np.testing.assert_allclose(nee, vx["tco2mic"] * 12e-3 * 3600.0 - inc[ix["aynpptot"]],
+                               rtol=1e-12, atol=1e-20)
+    # no biomass: NPP is (1 - rgrowth) x GPP summed over the PFTs
+    vx0 = dict(vx, cbiow=0 * vx["cbiow"], cbior=0 * vx["cbior"])
+    inc0 = OD.acc_increments(st, vx0, p)
+    gpp_from_an = dict(st, **{"canopy.ag_upper": st["canopy.an_upper"],
+                              "canopy.ag_lower": st["canopy.an_lower"]})
+    g = OD.acc_increments(gpp_from_an, vx0, p)[ix["aygpptot"]]
+    np.testing.assert_allclose(inc0[ix["aynpptot"]], 0.7 * g, rtol=1e-12, atol=1e-20)
+
+
+def test_compensated_sum_keeps_fp64_accuracy_in_fp32():
+    # one year of hourly increments summed in FP32 with the kernel's compensated add
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.0, 1e-3, 8760).astype(np.float32)
+    exact = math.fsum(float(v) for v in x)
+    s = np.float32(0.0)
+    c = np.float32(0.0)
+    naive = np.float32(0.0)
+    for v in x:
+        y = np.float32(v - c)
+        t = np.float32(s + y)
+        c = np.float32(np.float32(t - s) - y)
+        s = t
+        naive = np.float32(naive + v)
+    err_k = abs(float(s) - float(c) - exact) / exact
+    err_n = abs(float(naive) - exact) / exact              # the uncompensated FP32 sum drifts
+    assert err_k < 1e-7 and err_k * 20 < err_n, (err_k, err_n)
+
+
+# ---- host build of the driver source ------------------------------------------------------
+
+def _arena(fields, n, dtype):
+    return abi.pack(fields, n, dtype=dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_host_synthetic_forcing_matches_the_generator(dtype):
+    n, seed = 3000, 2024
+    f, slot, lat = synth.make_cells(n, seed)
+    a = _arena(f, n, dtype)
+    flags = abi.PREC_F64 if dtype == np.float64 else abi.PREC_MIXED
+    lib = _lib()
+    for hour in (0, 13, 30, 700):
+        rc = lib.ibis_hostcheck_forcing_synthetic(a.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                                  lat.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                                  C.c_uint64(seed), C.c_int64(0), C.c_int(172),
+                                                  C.c_int64(hour), C.c_int(flags))
+        assert rc == 0
+        got = abi.unpack(a, n)
+        ref = synth.forcing(n, seed, hour, lat, np.asarray(f["atm.psurf"], dtype).astype(np.float64))
+        for k, v in ref.items():
+            if k == "atm.psurf":
+                continue
+            r = np.asarray(v, dtype).astype(np.float64)
+            if dtype == np.float64:
+                np.testing.assert_allclose(got[k], r, rtol=1e-13, atol=1e-300, err_msg=k)
+            else:
+                np.testing.assert_allclose(got[k], r, rtol=2.5e-7, atol=1e-38, err_msg=k)
+
+
+def test_host_diurnal_matches_the_oracle():
+    n = 2000
+    dw = _daily(n)
+    lat = np.random.default_rng(9).uniform(-56.0, 84.0, n)
+    daily = np.stack([dw[k] for k in DR.DAILY_NAMES]).astype(np.float64).copy()
+    f, slot = workload(n, seed=1, hour=0, prec="f64")
+    a = _arena(f, n, np.float64)
+    lib = _lib()
+    for step in (0, 7, 12, 19):
+        t = step * 3600.0
+        rc = lib.ibis_hostcheck_forcing_diurnal(a.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                                daily.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                                lat.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                                C.c_double(t), C.c_int(172), C.c_double(3600.0),
+                                                C.c_uint64(77), C.c_int64(0), C.c_int64(step),
+                                                C.c_int(abi.PREC_F64))
+        assert rc == 0
+        got = abi.unpack(a, n)
+        rn = synth.uniform(77, 27, np.arange(n, dtype=np.uint64), step)
+        ref = OD.diurnal(t, 172.0, dw, lat, 3600.0, rn)
+        for k, v in ref.items():
+            np.testing.assert_allclose(got[k], v, rtol=1e-12, atol=1e-300, err_msg=k)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_host_accumulators_match_the_oracle(dtype):
+    n = 3000
+    st, vx, op = _acc_case(n)
+    st = {k: np.asarray(v, dtype).astype(np.float64) for k, v in st.items()}
+    vx = {k: np.asarray(v, dtype).astype(np.float64) for k, v in vx.items()}
+    a = _arena(st, n, dtype)
+    v = np.zeros((DR.NVEGX, n), dtype)
+    v[0:12] = vx["cbiow"].T
+    v[12:24] = vx["cbior"].T
+    v[24] = vx["sapfrac"]
+    v[25] = vx["tco2mic"]
+    acc = np.zeros((2 * DR.NACC, n), dtype)
+    p = DR.IbisAccParams()
+    p.rwood[:] = list(op.rwood)
+    p.rroot[:] = list(op.rroot)
+    p.rgrowth[:] = list(op.rgrowth)
+    for k in range(6):
+        p.froot[k][0], p.froot[k][1] = float(op.froot[k, 0]), float(op.froot[k, 1])
+    p.dtime, p.ndaypy = 3600.0, 365.0
+    flags = abi.PREC_F64 if dtype == np.float64 else abi.PREC_MIXED
+    lib = _lib()
+    for _ in range(3):
+        rc = lib.ibis_hostcheck_accumulate(a.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                           v.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                           acc.ctypes.data_as(C.c_void_p), C.c_int64(n),
+                                           C.byref(p), C.c_int64(n), C.c_int(flags))
+        assert rc == 0
+    ref = 3.0 * OD.acc_increments(st, vx, op)
+    tot = acc[:DR.NACC].astype(np.float64) - acc[DR.NACC:].astype(np.float64)
+    tol = 1e-12 if dtype == np.float64 else 3e-5
+    for j, name in enumerate(DR.ACC_NAMES):
+        scale = max(np.abs(ref[j]).max(), 1e-30)
+        assert np.abs(tot[j] - ref[j]).max() <= tol * scale, name
