@@ -8,7 +8,9 @@
 * ``ibis_amd/libibis_hostcheck.so``  -- the same step source built for the host with g++,
   loaded only by tests to check the kernel source against the oracle without a GPU.
 
-Both use ``-ffp-contract=off`` so the FP64 build follows the reference's rounding.
+Both use ``-ffp-contract=off`` so the FP64 build follows the reference's operation order; the
+HIP build also allows reciprocal division (see ``HIP_FLAGS``), so its results differ from the
+host build's in the last bits.
 """
 from __future__ import annotations
 
@@ -23,9 +25,18 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(ROOT, "include")
 
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+# Contraction into FMA stays off everywhere, so that the operation order is the reference's.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              f"-I{INC}"]
+# The mixed-precision and FP32 step kernels (ibis_step_fast.hip) may turn x / y into
+# x * (1 / y) with the hardware reciprocal and Newton steps (~1 ulp) and use the hardware's
+# FP32 divide and sqrt (~2.5 ulp): far inside those builds' tolerances, and it removes the
+# correctly rounded divide sequences that were a sixth to a third of their instructions.  The
+# FP64 kernels, the C ABI and the driver keep IEEE division.
+FAST_DIV = ["-freciprocal-math", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+SOURCES = (("ibis_step.hip", []), ("ibis_step_fast.hip", FAST_DIV), ("ibis_step_f64.hip", []),
+           ("ibis_driver.hip", []))
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
               "-Wno-unknown-pragmas", f"-I{INC}"]
 
@@ -50,8 +61,14 @@ def build(verbose: bool = False):
     check_layout_header()
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     out = os.path.join(HERE, "libibis_step.so")
-    msg = _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "ibis_step.hip"),
-                os.path.join(CSRC, "ibis_driver.hip"), "-o", out + ".tmp"])
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs, msg = [], ""
+    for name, extra in SOURCES:
+        obj = os.path.join(objdir, name.replace(".hip", ".o"))
+        msg += _run([hipcc, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, name), "-o", obj])
+        objs.append(obj)
+    msg += _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     hout = os.path.join(HERE, "libibis_hostcheck.so")
     msg2 = _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "ibis_hostcheck.cpp"), "-o", hout + ".tmp"])

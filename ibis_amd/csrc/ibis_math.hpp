@@ -31,9 +31,24 @@ IBIS_HD IBIS_INLINE double r_abs(double x) { return fabs(x); }
 IBIS_HD IBIS_INLINE float r_floor(float x) { return floorf(x); }
 IBIS_HD IBIS_INLINE double r_floor(double x) { return floor(x); }
 
-// jnp.maximum / jnp.minimum: a NaN in either operand gives NaN.
-template <class R> IBIS_HD IBIS_INLINE R mx(R a, R b) { return (a > b || a != a) ? a : b; }
-template <class R> IBIS_HD IBIS_INLINE R mn(R a, R b) { return (a < b || a != a) ? a : b; }
+// jnp.maximum / jnp.minimum: a NaN in either operand gives NaN (IEEE 754-2019 maximum and
+// minimum, +0 above -0).  On the device these are single instructions in FP32
+// (v_maximum3_f32 / v_minimum3_f32 on gfx950); the host build spells out the same result.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class R> IBIS_HD IBIS_INLINE R mx(R a, R b) { return __builtin_elementwise_maximum(a, b); }
+template <class R> IBIS_HD IBIS_INLINE R mn(R a, R b) { return __builtin_elementwise_minimum(a, b); }
+#else
+template <class R> IBIS_HD IBIS_INLINE R mx(R a, R b) {
+  if (a != a || b != b) return a + b;
+  if (a == b) return signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+template <class R> IBIS_HD IBIS_INLINE R mn(R a, R b) {
+  if (a != a || b != b) return a + b;
+  if (a == b) return signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+#endif
 template <class R> IBIS_HD IBIS_INLINE R clip(R x, R lo, R hi) { return mn(mx(x, lo), hi); }
 
 // x**n in the multiplication order of lax.integer_pow / jnp.power with an integer

@@ -1,0 +1,101 @@
+// Kernels of the hourly step and their launch sequence, included by the translation units
+// that instantiate them per precision: ibis_step_fast.hip (mixed and FP32, built with the
+// hardware reciprocal for division) and ibis_step_f64.hip (FP64, IEEE division throughout, so
+// that the FP64 build stays the GPU twin of the FP64 oracle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+
+#include "ibis_error.hpp"
+#include "ibis_step_cell.hpp"
+
+namespace ibis {
+
+constexpr int kBlock = 256;
+// Waves per SIMD each kernel is compiled for.  95,333 cells are 1,490 waves on 1,024 SIMDs,
+// so a kernel held to one wave per SIMD runs its waves in two rounds; surface and ground fit
+// two waves at the cost of a few spilled registers, which measures faster.  The canopy
+// kernel's three passes spill far more at two waves and stay at one.
+constexpr int kWavesSurface = 2, kWavesCanopy = 1, kWavesGround = 2;
+
+template <class R, class D>
+__device__ IBIS_INLINE Work<R, D> work_at(void* ws, int64_t pitch, int64_t i) {
+  D* d = (D*)ws;
+  R* r = (R*)(d + (int64_t)WD_N * pitch);
+  return Work<R, D>{d, r, pitch, i};
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock, kWavesSurface) void surface_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, IbisParams P, StepConst K,
+    int copy_in) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  if (copy_in) {
+#define X(F, W, OFF, ROLE)                                                      \
+  if ((ROLE) == IBIS_ROLE_IN) {                                                 \
+    for (int j = 0; j < (W); ++j)                                               \
+      out[(int64_t)((OFF) + j) * pitch + i] = in[(int64_t)((OFF) + j) * pitch + i]; \
+  }
+    IBIS_FIELD_TABLE
+#undef X
+  }
+  const Cell<S> c{in, out, pitch, i};
+  surface_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, const int8_t* slot,
+    int64_t slot_pitch, const double* table, IbisParams P, StepConst K) {
+  __shared__ R trow[IBIS_TYPE_ROWS];
+  for (int t = threadIdx.x; t < IBIS_TYPE_ROWS; t += blockDim.x) trow[t] = R(table[t]);
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  R vrow[IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
+    const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
+    vrow[j] = trow[row];
+  }
+  const Cell<S> c{in, out, pitch, i};
+  canopy_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), vrow, P, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock, kWavesGround) void ground_kernel(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, IbisParams P, StepConst K) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  const Cell<S> c{in, out, pitch, i};
+  ground_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
+}
+
+template <class S, class R, class D>
+inline int launch(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                  int64_t slot_pitch, const double* table, const IbisParams& P,
+                  const StepConst& K, int64_t ncell, int copy_in, hipStream_t st) {
+  const dim3 grid((unsigned)((ncell + kBlock - 1) / kBlock)), block(kBlock);
+  const S* pin = (const S*)in->planes;
+  S* pout = (S*)out->planes;
+  const int64_t pitch = in->pitch;
+  hipLaunchKernelGGL((surface_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     P, K, copy_in);
+  hipLaunchKernelGGL((canopy_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     slot, slot_pitch, table, P, K);
+  hipLaunchKernelGGL((ground_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     P, K);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(err_buf(), 512, "ibis_step: launch failed: %s", hipGetErrorString(e));
+    return IBIS_E_LAUNCH;
+  }
+  return IBIS_OK;
+}
+
+}  // namespace ibis
