@@ -72,10 +72,9 @@ def main():
     import torch
 
     from ibis_amd import layout as L, synth
-    from ibis_amd.params import default_sim_params, get_constants
+    from ibis_amd.params import default_sim_params, get_constants, root_fractions
     from ibis_amd.state import IBISState
     from ibis_amd import step as S
-    from oracle import params as OP
 
     n = args.cells
     hours = args.warmup + args.steps
@@ -83,7 +82,7 @@ def main():
     dtype = torch.float64 if args.precision == "f64" else torch.float32
     pitch = (n + 255) // 256 * 256
     fields, slot, lat = synth.make_cells(n, args.seed)
-    st = IBISState.from_numpy(fields, slot, synth.HSOI, OP.root_fractions(), device=dev,
+    st = IBISState.from_numpy(fields, slot, synth.HSOI, root_fractions(), device=dev,
                               dtype=dtype, pitch=pitch)
     consts, params = get_constants(), default_sim_params(bedrock=True)
     cparams = S.ibis_params(st, consts, params)

@@ -14,21 +14,6 @@ namespace ibis {
 constexpr int kDriverBlock = 256;
 
 template <class S>
-__device__ IBIS_INLINE void put_forcing(S* a, int64_t pitch, int64_t i, const Forcing& f) {
-  a[(int64_t)IBIS_P_ATM_TA * pitch + i] = S(f.ta);
-  a[(int64_t)IBIS_P_ATM_QA * pitch + i] = S(f.qa);
-  a[(int64_t)IBIS_P_ATM_UA * pitch + i] = S(f.ua);
-  a[(int64_t)IBIS_P_ATM_COSZEN * pitch + i] = S(f.coszen);
-  for (int b = 0; b < 2; ++b) {
-    a[(int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i] = S(f.solad[b]);
-    a[(int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i] = S(f.solai[b]);
-  }
-  a[(int64_t)IBIS_P_ATM_FIRA * pitch + i] = S(f.fira);
-  a[(int64_t)IBIS_P_ATM_RAINA * pitch + i] = S(f.raina);
-  a[(int64_t)IBIS_P_ATM_SNOWA * pitch + i] = S(f.snowa);
-}
-
-template <class S>
 __global__ __launch_bounds__(kDriverBlock) void forcing_synthetic_kernel(
     S* a, int64_t pitch, int64_t ncell, const double* lat, uint64_t seed, int64_t cell0,
     int jday0, int64_t hour) {
@@ -63,50 +48,7 @@ __global__ __launch_bounds__(kDriverBlock) void accumulate_kernel(
     int64_t ncell, IbisAccParams p) {
   const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
   if (i >= ncell) return;
-  auto ld = [&](int plane) { return st[(int64_t)plane * pitch + i]; };
-  AccIn<S> a;
-  for (int g = 0; g < 2; ++g) {
-    a.an_u[g] = ld(IBIS_P_CANOPY_AN_UPPER + g);
-    a.ag_u[g] = ld(IBIS_P_CANOPY_AG_UPPER + g);
-  }
-  for (int g = 0; g < 3; ++g) {
-    a.an_l[g] = ld(IBIS_P_CANOPY_AN_LOWER + g);
-    a.ag_l[g] = ld(IBIS_P_CANOPY_AG_LOWER + g);
-  }
-  a.lai_l = ld(IBIS_P_VEGETATION_LAI);
-  a.lai_u = ld(IBIS_P_VEGETATION_LAI + 1);
-  a.fu = ld(IBIS_P_VEGETATION_FU);
-  a.fl = ld(IBIS_P_VEGETATION_FL);
-  a.fi = ld(IBIS_P_SNOW_FI);
-  a.ts = ld(IBIS_P_CANOPY_TS);
-  for (int k = 0; k < IBIS_NSOILAY; ++k) a.tsoi[k] = ld(IBIS_P_SOIL_TSOI + k);
-  for (int j = 0; j < IBIS_NPFT; ++j) {
-    a.frac[j] = ld(IBIS_P_VEGETATION_FRAC + j);
-    a.cbiow[j] = vx[(int64_t)(IBIS_VEGX_CBIOW + j) * vpitch + i];
-    a.cbior[j] = vx[(int64_t)(IBIS_VEGX_CBIOR + j) * vpitch + i];
-  }
-  a.sapfrac = vx[(int64_t)IBIS_VEGX_SAPFRAC * vpitch + i];
-  a.tco2mic = vx[(int64_t)IBIS_VEGX_TCO2MIC * vpitch + i];
-  a.raina = ld(IBIS_P_ATM_RAINA);
-  a.snowa = ld(IBIS_P_ATM_SNOWA);
-  a.coszen = ld(IBIS_P_ATM_COSZEN);
-  a.ginvap = ld(IBIS_P_HYDROLOGY_GINVAP);
-  a.gsuvap = ld(IBIS_P_HYDROLOGY_GSUVAP);
-  a.gtrans = ld(IBIS_P_HYDROLOGY_GTRANS);
-  a.grunof = ld(IBIS_P_HYDROLOGY_GRUNOF);
-  a.gdrain = ld(IBIS_P_HYDROLOGY_GDRAIN);
-  a.fvapi = ld(IBIS_P_CANOPY_FVAPI);
-  S inc[IBIS_NACC];
-  acc_increments(inc, a, p);
-#pragma unroll
-  for (int j = 0; j < IBIS_NACC; ++j) {
-    S* sp = acc + (int64_t)j * apitch + i;
-    S* cp = acc + (int64_t)(IBIS_NACC + j) * apitch + i;
-    S s = *sp, c = *cp;
-    kahan_add(s, c, inc[j]);
-    *sp = s;
-    *cp = c;
-  }
+  accumulate_cell(st, pitch, vx, vpitch, acc, apitch, i, p);
 }
 
 static int launch_error(const char* what) {

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "ibis_driver.h"
+#include "ibis_layout.h"
 #include "ibis_math.hpp"
 
 namespace ibis {
@@ -271,6 +272,72 @@ IBIS_HD IBIS_INLINE void acc_increments(R inc[IBIS_NACC], const AccIn<R>& a,
   const bool night = a.coszen <= R(0);
   inc[IBIS_ACC_NEE_NIGHT] = night ? nee : R(0);
   inc[IBIS_ACC_NEE_DAY] = night ? R(0) : nee;
+}
+
+// Store one cell-hour of forcing into the atm planes of a state arena.
+template <class S>
+IBIS_HD IBIS_INLINE void put_forcing(S* a, int64_t pitch, int64_t i, const Forcing& f) {
+  a[(int64_t)IBIS_P_ATM_TA * pitch + i] = S(f.ta);
+  a[(int64_t)IBIS_P_ATM_QA * pitch + i] = S(f.qa);
+  a[(int64_t)IBIS_P_ATM_UA * pitch + i] = S(f.ua);
+  a[(int64_t)IBIS_P_ATM_COSZEN * pitch + i] = S(f.coszen);
+  for (int b = 0; b < 2; ++b) {
+    a[(int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i] = S(f.solad[b]);
+    a[(int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i] = S(f.solai[b]);
+  }
+  a[(int64_t)IBIS_P_ATM_FIRA * pitch + i] = S(f.fira);
+  a[(int64_t)IBIS_P_ATM_RAINA * pitch + i] = S(f.raina);
+  a[(int64_t)IBIS_P_ATM_SNOWA * pitch + i] = S(f.snowa);
+}
+
+// Add one step of cell i to the accumulators: read the state after the step and the
+// vegetation extras, form the increments, Kahan-add them into the sum and compensation planes.
+template <class S>
+IBIS_HD IBIS_INLINE void accumulate_cell(const S* st, int64_t pitch, const S* vx, int64_t vpitch,
+                                         S* acc, int64_t apitch, int64_t i,
+                                         const IbisAccParams& p) {
+  AccIn<S> a;
+  auto ld = [&](int plane) { return st[(int64_t)plane * pitch + i]; };
+  for (int g = 0; g < 2; ++g) {
+    a.an_u[g] = ld(IBIS_P_CANOPY_AN_UPPER + g);
+    a.ag_u[g] = ld(IBIS_P_CANOPY_AG_UPPER + g);
+  }
+  for (int g = 0; g < 3; ++g) {
+    a.an_l[g] = ld(IBIS_P_CANOPY_AN_LOWER + g);
+    a.ag_l[g] = ld(IBIS_P_CANOPY_AG_LOWER + g);
+  }
+  a.lai_l = ld(IBIS_P_VEGETATION_LAI);
+  a.lai_u = ld(IBIS_P_VEGETATION_LAI + 1);
+  a.fu = ld(IBIS_P_VEGETATION_FU);
+  a.fl = ld(IBIS_P_VEGETATION_FL);
+  a.fi = ld(IBIS_P_SNOW_FI);
+  a.ts = ld(IBIS_P_CANOPY_TS);
+  for (int k = 0; k < IBIS_NSOILAY; ++k) a.tsoi[k] = ld(IBIS_P_SOIL_TSOI + k);
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    a.frac[j] = ld(IBIS_P_VEGETATION_FRAC + j);
+    a.cbiow[j] = vx[(int64_t)(IBIS_VEGX_CBIOW + j) * vpitch + i];
+    a.cbior[j] = vx[(int64_t)(IBIS_VEGX_CBIOR + j) * vpitch + i];
+  }
+  a.sapfrac = vx[(int64_t)IBIS_VEGX_SAPFRAC * vpitch + i];
+  a.tco2mic = vx[(int64_t)IBIS_VEGX_TCO2MIC * vpitch + i];
+  a.raina = ld(IBIS_P_ATM_RAINA);
+  a.snowa = ld(IBIS_P_ATM_SNOWA);
+  a.coszen = ld(IBIS_P_ATM_COSZEN);
+  a.ginvap = ld(IBIS_P_HYDROLOGY_GINVAP);
+  a.gsuvap = ld(IBIS_P_HYDROLOGY_GSUVAP);
+  a.gtrans = ld(IBIS_P_HYDROLOGY_GTRANS);
+  a.grunof = ld(IBIS_P_HYDROLOGY_GRUNOF);
+  a.gdrain = ld(IBIS_P_HYDROLOGY_GDRAIN);
+  a.fvapi = ld(IBIS_P_CANOPY_FVAPI);
+  S inc[IBIS_NACC];
+  acc_increments(inc, a, p);
+#pragma unroll
+  for (int j = 0; j < IBIS_NACC; ++j) {
+    S s = acc[(int64_t)j * apitch + i], c = acc[(int64_t)(IBIS_NACC + j) * apitch + i];
+    kahan_add(s, c, inc[j]);
+    acc[(int64_t)j * apitch + i] = s;
+    acc[(int64_t)(IBIS_NACC + j) * apitch + i] = c;
+  }
 }
 
 }  // namespace ibis

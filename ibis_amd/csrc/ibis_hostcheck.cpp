@@ -84,66 +84,9 @@ extern "C" int ibis_hostcheck_step(const void* in, void* out, int64_t pitch, int
 namespace {
 
 template <class S>
-void put_forcing_host(S* a, int64_t pitch, int64_t i, const ibis::Forcing& f) {
-  a[(int64_t)IBIS_P_ATM_TA * pitch + i] = S(f.ta);
-  a[(int64_t)IBIS_P_ATM_QA * pitch + i] = S(f.qa);
-  a[(int64_t)IBIS_P_ATM_UA * pitch + i] = S(f.ua);
-  a[(int64_t)IBIS_P_ATM_COSZEN * pitch + i] = S(f.coszen);
-  for (int b = 0; b < 2; ++b) {
-    a[(int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i] = S(f.solad[b]);
-    a[(int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i] = S(f.solai[b]);
-  }
-  a[(int64_t)IBIS_P_ATM_FIRA * pitch + i] = S(f.fira);
-  a[(int64_t)IBIS_P_ATM_RAINA * pitch + i] = S(f.raina);
-  a[(int64_t)IBIS_P_ATM_SNOWA * pitch + i] = S(f.snowa);
-}
-
-template <class S>
 void accumulate_host(const S* st, int64_t pitch, const S* vx, int64_t vpitch, S* acc,
                      int64_t apitch, int64_t ncell, const IbisAccParams& p) {
-  for (int64_t i = 0; i < ncell; ++i) {
-    auto ld = [&](int plane) { return st[(int64_t)plane * pitch + i]; };
-    ibis::AccIn<S> a;
-    for (int g = 0; g < 2; ++g) {
-      a.an_u[g] = ld(IBIS_P_CANOPY_AN_UPPER + g);
-      a.ag_u[g] = ld(IBIS_P_CANOPY_AG_UPPER + g);
-    }
-    for (int g = 0; g < 3; ++g) {
-      a.an_l[g] = ld(IBIS_P_CANOPY_AN_LOWER + g);
-      a.ag_l[g] = ld(IBIS_P_CANOPY_AG_LOWER + g);
-    }
-    a.lai_l = ld(IBIS_P_VEGETATION_LAI);
-    a.lai_u = ld(IBIS_P_VEGETATION_LAI + 1);
-    a.fu = ld(IBIS_P_VEGETATION_FU);
-    a.fl = ld(IBIS_P_VEGETATION_FL);
-    a.fi = ld(IBIS_P_SNOW_FI);
-    a.ts = ld(IBIS_P_CANOPY_TS);
-    for (int k = 0; k < IBIS_NSOILAY; ++k) a.tsoi[k] = ld(IBIS_P_SOIL_TSOI + k);
-    for (int j = 0; j < IBIS_NPFT; ++j) {
-      a.frac[j] = ld(IBIS_P_VEGETATION_FRAC + j);
-      a.cbiow[j] = vx[(int64_t)(IBIS_VEGX_CBIOW + j) * vpitch + i];
-      a.cbior[j] = vx[(int64_t)(IBIS_VEGX_CBIOR + j) * vpitch + i];
-    }
-    a.sapfrac = vx[(int64_t)IBIS_VEGX_SAPFRAC * vpitch + i];
-    a.tco2mic = vx[(int64_t)IBIS_VEGX_TCO2MIC * vpitch + i];
-    a.raina = ld(IBIS_P_ATM_RAINA);
-    a.snowa = ld(IBIS_P_ATM_SNOWA);
-    a.coszen = ld(IBIS_P_ATM_COSZEN);
-    a.ginvap = ld(IBIS_P_HYDROLOGY_GINVAP);
-    a.gsuvap = ld(IBIS_P_HYDROLOGY_GSUVAP);
-    a.gtrans = ld(IBIS_P_HYDROLOGY_GTRANS);
-    a.grunof = ld(IBIS_P_HYDROLOGY_GRUNOF);
-    a.gdrain = ld(IBIS_P_HYDROLOGY_GDRAIN);
-    a.fvapi = ld(IBIS_P_CANOPY_FVAPI);
-    S inc[IBIS_NACC];
-    ibis::acc_increments(inc, a, p);
-    for (int j = 0; j < IBIS_NACC; ++j) {
-      S s = acc[(int64_t)j * apitch + i], c = acc[(int64_t)(IBIS_NACC + j) * apitch + i];
-      ibis::kahan_add(s, c, inc[j]);
-      acc[(int64_t)j * apitch + i] = s;
-      acc[(int64_t)(IBIS_NACC + j) * apitch + i] = c;
-    }
-  }
+  for (int64_t i = 0; i < ncell; ++i) ibis::accumulate_cell(st, pitch, vx, vpitch, acc, apitch, i, p);
 }
 
 }  // namespace
@@ -157,8 +100,8 @@ extern "C" int ibis_hostcheck_forcing_synthetic(void* state, int64_t pitch, cons
     const double ps = f64 ? ((double*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i]
                           : (double)((float*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i];
     ibis::synthetic_forcing(f, lat[i], ps, seed, (uint64_t)(cell0 + i), jday0, hour);
-    if (f64) put_forcing_host((double*)state, pitch, i, f);
-    else put_forcing_host((float*)state, pitch, i, f);
+    if (f64) ibis::put_forcing((double*)state, pitch, i, f);
+    else ibis::put_forcing((float*)state, pitch, i, f);
   }
   return 0;
 }
@@ -180,10 +123,10 @@ extern "C" int ibis_hostcheck_forcing_diurnal(void* state, int64_t pitch, const 
     double ps;
     ibis::diurnal_forcing(f, ps, dw, lat[i], time, (double)jday, dtime, rn);
     if (f64) {
-      put_forcing_host((double*)state, pitch, i, f);
+      ibis::put_forcing((double*)state, pitch, i, f);
       ((double*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = ps;
     } else {
-      put_forcing_host((float*)state, pitch, i, f);
+      ibis::put_forcing((float*)state, pitch, i, f);
       ((float*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = (float)ps;
     }
   }

@@ -1,0 +1,161 @@
+// HIP kernels and C ABI of the served step (include/ibis_serve.h): one hour of a packed
+// batch of requests per launch sequence.
+//
+// 64-thread blocks: one wavefront, one request.  The request index of a block is uniform,
+// so the request's parameters, constants and type table are read with scalar loads as the
+// plain step reads its by-value parameters.  The per-cell arithmetic is the plain step's
+// (ibis_step_cell.hpp), so a request steps the same alone or in any mix.  Built with the
+// mixed-precision kernels' division flags.
+#include <hip/hip_runtime.h>
+
+#include "ibis_driver_cell.hpp"
+#include "ibis_error.hpp"
+#include "ibis_serve.h"
+#include "ibis_step_kernels.hpp"
+
+namespace ibis {
+
+constexpr int kServe = IBIS_SERVE_BLOCK;
+
+// The block's request (uniform across the block), or null when the block is idle or its
+// request has finished.  Lanes past the request's last cell are tested separately, so that
+// the pointer stays uniform and the request is read with scalar loads.
+__device__ IBIS_INLINE const IbisRequest* serve_request(const IbisRequest* __restrict__ reqs,
+                                                        const int32_t* __restrict__ blk) {
+  const int r = blk[blockIdx.x];
+  if (r < 0) return nullptr;
+  const IbisRequest* q = reqs + r;
+  return q->hours_left > 0 ? q : nullptr;
+}
+
+#define SERVE_CELL(q, i)                                              \
+  const IbisRequest* q = serve_request(reqs, blk);                    \
+  if (!q) return;                                                     \
+  const int64_t i = (int64_t)blockIdx.x * kServe + threadIdx.x;       \
+  if (i >= q->start + q->ncell) return
+
+__device__ IBIS_INLINE StepConst serve_const(const IbisRequest& q) {
+  StepConst K;
+  K.dtime = q.dtime;
+  K.tfac = q.tfac;
+  K.zweight = q.zweight;
+  K.nytimes_new = q.nytimes + 1.0;
+  K.bedrock = q.bedrock;
+  return K;
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kServe) __attribute__((amdgpu_waves_per_eu(kWavesSurface)))
+void serve_surface_kernel(S* a, void* ws, int64_t pitch, const double* __restrict__ lat,
+                          const IbisRequest* __restrict__ reqs,
+                          const int32_t* __restrict__ blk) {
+  SERVE_CELL(q, i);
+  Forcing f;
+  synthetic_forcing(f, lat[i], (double)a[(int64_t)IBIS_P_ATM_PSURF * pitch + i], q->seed,
+                    (uint64_t)(i - q->start), q->jday0, q->hour);
+  put_forcing(a, pitch, i, f);
+  const StepConst K = serve_const(*q);
+  const Cell<S> c{a, a, pitch, i};
+  surface_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), q->params, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kServe) __attribute__((amdgpu_waves_per_eu(kWavesCanopy)))
+void serve_canopy_kernel(S* a, void* ws, int64_t pitch, const int8_t* __restrict__ slot,
+                         const IbisRequest* __restrict__ reqs,
+                         const int32_t* __restrict__ blk) {
+  __shared__ R trow[IBIS_TYPE_ROWS];
+  const IbisRequest* q = serve_request(reqs, blk);
+  if (!q) return;
+  for (int t = threadIdx.x; t < IBIS_TYPE_ROWS; t += kServe) trow[t] = R(q->type_table[t]);
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kServe + threadIdx.x;
+  if (i >= q->start + q->ncell) return;
+  R vrow[IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    const int s = (int)slot[(int64_t)j * pitch + i];
+    const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
+    vrow[j] = trow[row];
+  }
+  const StepConst K = serve_const(*q);
+  const Cell<S> c{a, a, pitch, i};
+  canopy_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), vrow, q->params, K);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kServe) __attribute__((amdgpu_waves_per_eu(kWavesGround)))
+void serve_ground_kernel(S* a, void* ws, int64_t pitch, const IbisRequest* __restrict__ reqs,
+                         const int32_t* __restrict__ blk) {
+  SERVE_CELL(q, i);
+  const StepConst K = serve_const(*q);
+  const Cell<S> c{a, a, pitch, i};
+  ground_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), q->params, K);
+}
+
+template <class S>
+__global__ __launch_bounds__(kServe) void serve_accumulate_kernel(
+    const S* a, int64_t pitch, const S* vx, S* acc, IbisAccParams p,
+    const IbisRequest* __restrict__ reqs, const int32_t* __restrict__ blk) {
+  SERVE_CELL(q, i);
+  accumulate_cell(a, pitch, vx, pitch, acc, pitch, i, p);
+}
+
+// One thread per request: advance the hour of every request that ran this tick.
+__global__ void serve_advance_kernel(IbisRequest* reqs, int64_t nreq) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nreq) return;
+  IbisRequest& q = reqs[r];
+  if (q.hours_left <= 0) return;
+  q.hour = q.hour + 1;
+  q.nytimes = q.nytimes + 1.0;
+  q.hours_left = q.hours_left - 1;
+}
+
+}  // namespace ibis
+
+extern "C" int ibis_serve_tick(const IbisArena* arena, void* workspace, int64_t workspace_bytes,
+                               const int8_t* slot_type, const double* lat_deg,
+                               const IbisArena* acc, const IbisArena* vegx,
+                               const IbisAccParams* acc_params, IbisRequest* reqs, int64_t nreq,
+                               const int32_t* block_req, int64_t nblock, void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (!arena || !arena->planes || !workspace || !slot_type || !lat_deg || !reqs || !block_req)
+    return fail(IBIS_E_ARG, "ibis_serve_tick: null pointer argument");
+  const int64_t pitch = arena->pitch;
+  if (nreq < 0 || nblock < 0 || nblock * kServe > pitch)
+    return fail(IBIS_E_ARG, "ibis_serve_tick: negative counts or nblock * 64 > pitch");
+  if (workspace_bytes < work_bytes(pitch, (int)sizeof(float), (int)sizeof(double)))
+    return fail(IBIS_E_ARG, "ibis_serve_tick: workspace smaller than "
+                            "ibis_step_workspace_bytes(pitch, IBIS_PREC_MIXED)");
+  if (((uintptr_t)workspace & 255u) != 0)
+    return fail(IBIS_E_ARG, "ibis_serve_tick: workspace must be 256-byte aligned");
+  const bool with_acc = acc != nullptr;
+  if (with_acc && (!acc->planes || acc->pitch != pitch || !vegx || !vegx->planes ||
+                   vegx->pitch != pitch || !acc_params))
+    return fail(IBIS_E_ARG, "ibis_serve_tick: accumulators need acc, vegx and params of the "
+                            "arena's pitch");
+  if (nblock == 0 || nreq == 0) return IBIS_OK;
+  hipStream_t s = (hipStream_t)hip_stream;
+  float* a = (float*)arena->planes;
+  const dim3 grid((unsigned)nblock), block(kServe);
+  hipLaunchKernelGGL((serve_surface_kernel<float, float, double>), grid, block, 0, s, a,
+                     workspace, pitch, lat_deg, (const IbisRequest*)reqs, block_req);
+  hipLaunchKernelGGL((serve_canopy_kernel<float, float, double>), grid, block, 0, s, a,
+                     workspace, pitch, slot_type, (const IbisRequest*)reqs, block_req);
+  hipLaunchKernelGGL((serve_ground_kernel<float, float, double>), grid, block, 0, s, a,
+                     workspace, pitch, (const IbisRequest*)reqs, block_req);
+  if (with_acc)
+    hipLaunchKernelGGL((serve_accumulate_kernel<float>), grid, block, 0, s, (const float*)a,
+                       pitch, (const float*)vegx->planes, (float*)acc->planes, *acc_params,
+                       (const IbisRequest*)reqs, block_req);
+  hipLaunchKernelGGL(serve_advance_kernel, dim3((unsigned)((nreq + 255) / 256)), dim3(256), 0,
+                     s, reqs, nreq);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(err_buf(), 512, "ibis_serve_tick: launch failed: %s", hipGetErrorString(e));
+    return IBIS_E_LAUNCH;
+  }
+  return IBIS_OK;
+}

@@ -165,3 +165,21 @@ class SnowConstants:
     hsnomin: float = 0.15
     fimin: float = 0.00004
     fimax: float = 1.00
+
+
+#: the reference's default soil layering above bedrock, m (python-port/ibis/io/soil_init.py:17)
+SOIL_LAYERS = (0.10, 0.15, 0.25, 0.50, 1.00, 2.00)
+
+
+def root_fractions(hsoi=SOIL_LAYERS, beta_lower: float = 0.950, beta_upper: float = 0.975):
+    """Root fraction per soil layer, (nsoilay, 2): column 0 lower canopy, 1 upper canopy.
+
+    Jackson et al. (1996) asymptotic profile 1 - beta^d at the layer bottoms (d in cm),
+    normalised to the column (python-port/ibis/io/veg_init.py:78-103)."""
+    depth = np.cumsum(np.asarray(hsoi, dtype=np.float64)) * 100.0
+    out = np.zeros((len(depth), 2))
+    for col, beta in ((0, beta_lower), (1, beta_upper)):
+        cum = 1.0 - beta ** depth
+        prev = np.concatenate([[0.0], cum[:-1]])
+        out[:, col] = (cum - prev) / (1.0 - beta ** depth[-1])
+    return out

@@ -1,0 +1,440 @@
+"""Served step: independent runs ("requests") of the hourly step packed into one arena and
+stepped together, with continuous batching.
+
+Each request brings its own cells (seeded synthetic state), its own parameter vector, its
+own forcing seed and its own length in hours.  The engine keeps every admitted request's
+state resident in HBM; once per tick (one hour) it steps all active requests with one launch
+sequence (``ibis_serve_tick`` of ``include/ibis_serve.h``: forcing, the step's three phases,
+the accumulators, the hour advance), optionally replayed from a HIP graph captured once.
+Between ticks it retires finished requests (their results are copied to pinned host memory
+asynchronously and their blocks freed) and admits waiting ones into free blocks, in arrival
+order, as long as they fit: admission is by memory footprint.
+
+A request's cells occupy a 64-cell-aligned range; every 64-cell block belongs to one
+request, so the per-cell arithmetic is exactly the plain step's and a request's result does
+not depend on what else is in the batch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import abi
+from . import driver as DR
+from . import layout as L
+from . import synth
+from .params import SnowConstants, default_sim_params, get_constants, root_fractions
+
+BLOCK = 64
+
+
+class IbisRequest(C.Structure):
+    _fields_ = [("params", abi.IbisParams), ("type_table", C.c_double * abi.TYPE_ROWS),
+                ("dtime", C.c_double), ("tfac", C.c_double), ("zweight", C.c_double),
+                ("nytimes", C.c_double), ("start", C.c_int64), ("ncell", C.c_int64),
+                ("hour", C.c_int64), ("hours_left", C.c_int64), ("seed", C.c_uint64),
+                ("jday0", C.c_int32), ("bedrock", C.c_int32)]
+
+
+#: parameters drawn per request within +-20 % of their defaults (with the 12 per-PFT Vmax
+#: of the type table): quantum efficiencies, colimitation curvatures, stomatal slope and
+#: intercept, soil-water stress, leaf / stem heat capacities, interception capacities,
+#: drainage and puddle limits, bedrock conductivity and heat capacity
+PERTURBED = ("alpha3", "alpha4", "theta3", "theta4", "beta3", "beta4", "coefm", "coefb",
+             "stressfac", "cleaf", "cstem", "cgrass", "chu", "chs", "chl", "wliqumax",
+             "wsnoumax", "fmax", "bperm", "wpudmax", "bed_consoi", "bed_csoi")
+
+
+def _lib_serve():
+    lib = _lib.load()
+    if not getattr(lib, "_serve_typed", False):
+        lib.ibis_serve_tick.argtypes = [C.POINTER(abi.IbisArena), C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_void_p, C.POINTER(abi.IbisArena),
+                                        C.POINTER(abi.IbisArena), C.POINTER(DR.IbisAccParams),
+                                        C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                        C.c_void_p]
+        lib.ibis_serve_tick.restype = C.c_int
+        lib._serve_typed = True
+    return lib
+
+
+@dataclass
+class Request:
+    """One run: ``ncell`` seeded cells stepped ``hours`` hours with its own parameters."""
+    rid: int
+    ncell: int
+    hours: int
+    seed: int
+    perturb: bool = True
+    jday0: int = 172
+    arrival: float = 0.0            # seconds since the stream's start
+    # filled by the engine
+    slot: int = -1
+    start: int = -1
+    admitted: float = float("nan")
+    done: float = float("nan")
+    ticks_left: int = 0
+    result: dict = field(default_factory=dict)
+    _copy: object = None
+    _event: object = None
+
+
+def _draw(seed, k, n=1):
+    u = synth.uniform(seed, 500 + k, np.arange(n, dtype=np.uint64), 0)
+    return 0.8 + 0.4 * u
+
+
+def make_request_params(req: Request, froot, consts=None, sim=None):
+    consts = consts or get_constants()
+    sim = sim or default_sim_params(bedrock=True)
+    p = abi.params_from(consts, sim, SnowConstants(), synth.HSOI, froot, 380.0e-6)
+    vmax = np.asarray(sim.stomata_params.vmax_pft, dtype=np.float64).copy()
+    if req.perturb:
+        for k, name in enumerate(PERTURBED):
+            f = getattr(p, name)
+            if isinstance(f, float):
+                setattr(p, name, f * float(_draw(req.seed, k)[0]))
+            else:
+                m = _draw(req.seed, k, len(f))
+                for j in range(len(f)):
+                    f[j] = f[j] * float(m[j])
+        vmax = vmax * _draw(req.seed, 100, L.NPFT)
+    return p, synth.type_table(vmax)
+
+
+class ServeEngine:
+    """Packed arena of ``capacity`` cells and a table of ``max_requests`` request records."""
+
+    def __init__(self, capacity: int, max_requests: int = 4096, device="cuda",
+                 accumulate: bool = True, dtime: float = 3600.0, use_graph: bool = True):
+        self.dev = torch.device(device)
+        self.pitch = int(math.ceil(capacity / 256.0) * 256)
+        self.nblock = self.pitch // BLOCK
+        self.max_requests = int(max_requests)
+        self.dtime = float(dtime)
+        self.consts, self.sim = get_constants(), default_sim_params(bedrock=True)
+        self.froot = root_fractions()
+        self.arena = torch.zeros((L.NPLANE, self.pitch), dtype=torch.float32, device=self.dev)
+        self.slot = torch.full((L.NPFT, self.pitch), -1, dtype=torch.int8, device=self.dev)
+        self.lat = torch.zeros(self.pitch, dtype=torch.float64, device=self.dev)
+        lib = _lib_serve()
+        wsb = int(lib.ibis_step_workspace_bytes(C.c_int64(self.pitch), C.c_int(abi.PREC_MIXED)))
+        self.ws = torch.empty(wsb, dtype=torch.uint8, device=self.dev)
+        self.accumulate = accumulate
+        self.acc = torch.zeros((2 * DR.NACC, self.pitch), dtype=torch.float32, device=self.dev)
+        self.vegx = torch.zeros((DR.NVEGX, self.pitch), dtype=torch.float32, device=self.dev)
+        self.accp = DR.IbisAccParams()
+        r = DR.RespirationParams()
+        self.accp.rwood[:] = list(r.rwood)
+        self.accp.rroot[:] = list(r.rroot)
+        self.accp.rgrowth[:] = list(r.rgrowth)
+        for k in range(L.NSOILAY):
+            self.accp.froot[k][0], self.accp.froot[k][1] = float(self.froot[k, 0]), float(self.froot[k, 1])
+        self.accp.dtime, self.accp.ndaypy = self.dtime, r.ndaypy
+        self.recsize = C.sizeof(IbisRequest)
+        self.reqs = torch.zeros(self.max_requests * self.recsize, dtype=torch.uint8, device=self.dev)
+        self.blk = torch.full((self.nblock,), -1, dtype=torch.int32, device=self.dev)
+        self.stream = torch.cuda.Stream(device=self.dev)
+        self.free_slots = list(range(self.max_requests))[::-1]
+        self.free_ranges = [(0, self.nblock)]          # (first block, blocks), sorted
+        self.active: list[Request] = []
+        self.retiring: list[Request] = []
+        self.use_graph = use_graph
+        self.graphs = {}                                # launched blocks -> captured tick
+        self.ticks = 0
+        self.cell_hours = 0
+
+    # -- memory -----------------------------------------------------------------------------
+    @staticmethod
+    def bytes_per_cell():
+        """HBM bytes one cell of capacity costs: arena, workspace, slots, latitude,
+        accumulators and vegetation extras."""
+        return (L.NPLANE * 4 + (34 * 8 + 25 * 4) + L.NPFT + 8 + 2 * DR.NACC * 4 + DR.NVEGX * 4)
+
+    def _alloc(self, nblocks):
+        for k, (b0, nb) in enumerate(self.free_ranges):
+            if nb >= nblocks:
+                if nb == nblocks:
+                    self.free_ranges.pop(k)
+                else:
+                    self.free_ranges[k] = (b0 + nblocks, nb - nblocks)
+                return b0
+        return -1
+
+    def _free(self, b0, nblocks):
+        self.free_ranges.append((b0, nblocks))
+        self.free_ranges.sort()
+        merged = []
+        for r in self.free_ranges:
+            if merged and merged[-1][0] + merged[-1][1] == r[0]:
+                merged[-1] = (merged[-1][0], merged[-1][1] + r[1])
+            else:
+                merged.append(r)
+        self.free_ranges = merged
+
+    # -- admission ----------------------------------------------------------------------------
+    def admit(self, req: Request, now: float = 0.0) -> bool:
+        nb = -(-req.ncell // BLOCK)
+        if not self.free_slots:
+            return False
+        b0 = self._alloc(nb)
+        if b0 < 0:
+            return False
+        slot = self.free_slots.pop()
+        start, n = b0 * BLOCK, req.ncell
+        f, st, lat = synth.make_cells(n, req.seed)
+        with torch.cuda.stream(self.stream):
+            host = torch.from_numpy(abi.pack(f, n, dtype=np.float32)).pin_memory()
+            self.arena[:, start:start + n].copy_(host.to(self.dev, non_blocking=True))
+            self.slot[:, start:start + n].copy_(torch.from_numpy(np.ascontiguousarray(st.T)).to(self.dev))
+            self.lat[start:start + n].copy_(torch.from_numpy(lat).to(self.dev))
+            self.acc[:, start:start + nb * BLOCK].zero_()
+            p, table = make_request_params(req, self.froot, self.consts, self.sim)
+            rec = IbisRequest()
+            rec.params = p
+            rec.type_table[:] = [float(x) for x in table]
+            rec.dtime, rec.tfac = self.dtime, 1.0 / (0.999 ** p.cappa)
+            rec.zweight = math.exp(-1.0 / (10.0 * 86400.0 / self.dtime))
+            rec.nytimes = 0.0
+            rec.start, rec.ncell, rec.hour, rec.hours_left = start, n, 0, req.hours
+            rec.seed, rec.jday0, rec.bedrock = req.seed, req.jday0, 1
+            rb = torch.frombuffer(bytearray(bytes(rec)), dtype=torch.uint8)
+            self.reqs[slot * self.recsize:(slot + 1) * self.recsize].copy_(rb.to(self.dev))
+            self.blk[b0:b0 + nb].fill_(slot)
+        req.slot, req.start, req.ticks_left, req.admitted = slot, start, req.hours, now
+        self.active.append(req)
+        return True
+
+    # -- ticks --------------------------------------------------------------------------------
+    GRID_BUCKET = 4096                                  # blocks (262,144 cells)
+
+    def _grid(self):
+        """Blocks to launch: up to the highest allocated block, rounded up to a bucket, so
+        that a tick does not dispatch the idle tail of the capacity."""
+        top = 0
+        for r in self.active:
+            top = max(top, (r.start + r.ncell + BLOCK - 1) // BLOCK)
+        return min(self.nblock, -(-max(top, 1) // self.GRID_BUCKET) * self.GRID_BUCKET)
+
+    def _launch(self, stream, nblock):
+        lib = _lib_serve()
+        a = abi.IbisArena(self.arena.data_ptr(), self.pitch)
+        acc = abi.IbisArena(self.acc.data_ptr(), self.pitch)
+        vx = abi.IbisArena(self.vegx.data_ptr(), self.pitch)
+        rc = lib.ibis_serve_tick(C.byref(a), C.c_void_p(self.ws.data_ptr()),
+                                 C.c_int64(self.ws.numel()), C.c_void_p(self.slot.data_ptr()),
+                                 C.c_void_p(self.lat.data_ptr()),
+                                 C.byref(acc) if self.accumulate else None,
+                                 C.byref(vx) if self.accumulate else None, C.byref(self.accp),
+                                 C.c_void_p(self.reqs.data_ptr()), C.c_int64(self.max_requests),
+                                 C.c_void_p(self.blk.data_ptr()), C.c_int64(nblock),
+                                 C.c_void_p(stream.cuda_stream))
+        _lib.check(rc, "ibis_serve_tick")
+
+    def tick(self):
+        """One hour of every active request (enqueued, not waited for)."""
+        nb = self._grid()
+        if self.use_graph:
+            g = self.graphs.get(nb)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=self.stream):
+                    self._launch(torch.cuda.current_stream(), nb)
+                self.graphs[nb] = g
+            with torch.cuda.stream(self.stream):
+                g.replay()
+        else:
+            self._launch(self.stream, nb)
+        self.ticks += 1
+        for r in self.active:
+            self.cell_hours += r.ncell
+            r.ticks_left -= 1
+
+    # -- retirement ---------------------------------------------------------------------------
+    RESULT_KEYS = ("soil.tsoi", "soil.wsoi", "snow.fi", "canopy.tu")
+
+    def retire_finished(self):
+        """Start the result copies of requests whose hours are done and free their blocks."""
+        done = [r for r in self.active if r.ticks_left <= 0]
+        if not done:
+            return
+        self.active = [r for r in self.active if r.ticks_left > 0]
+        with torch.cuda.stream(self.stream):
+            for r in done:
+                nb = -(-r.ncell // BLOCK)
+                s, n = r.start, r.ncell
+                width = {f.key: f.width for f in L.FIELDS}
+                planes = [L.PLANE_OFFSET[k] + j for k in self.RESULT_KEYS
+                          for j in range(width[k])]
+                idx = torch.as_tensor(planes, device=self.dev)
+                res = torch.cat([self.arena[:, s:s + n].index_select(0, idx),
+                                 (self.acc[:DR.NACC, s:s + n] - self.acc[DR.NACC:, s:s + n])])
+                host = torch.empty(res.shape, dtype=res.dtype, pin_memory=True)
+                host.copy_(res, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+                r._copy, r._event = host, ev
+                self.blk[s // BLOCK:s // BLOCK + nb].fill_(-1)
+                self._free(s // BLOCK, nb)
+                self.free_slots.append(r.slot)
+                self.retiring.append(r)
+
+    def collect(self, now_fn, block=False):
+        """Requests whose result copies have landed; stamps their completion time."""
+        out, keep = [], []
+        for r in self.retiring:
+            if block:
+                r._event.synchronize()
+            if r._event.query():
+                r.done = now_fn()
+                r.result = {"state_acc": r._copy.numpy()}
+                r._copy = r._event = None
+                out.append(r)
+            else:
+                keep.append(r)
+        self.retiring = keep
+        return out
+
+    def sync(self):
+        self.stream.synchronize()
+
+
+def run_alone_and_mixed(reqs, capacity, use_graph=True):
+    """Results of each request stepped alone and all of them stepped together."""
+    def run(batch):
+        eng = ServeEngine(capacity, max_requests=64, use_graph=use_graph)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        for r in batch:
+            assert eng.admit(r, now())
+        out = {}
+        while eng.active or eng.retiring:
+            if eng.active:
+                eng.tick()
+            eng.retire_finished()
+            for r in eng.collect(now, block=not eng.active):
+                out[r.rid] = r.result["state_acc"]
+        return out
+    alone = {}
+    for r in reqs:
+        alone.update(run([Request(r.rid, r.ncell, r.hours, r.seed, r.perturb, r.jday0)]))
+    mixed = run([Request(r.rid, r.ncell, r.hours, r.seed, r.perturb, r.jday0) for r in reqs])
+    return alone, mixed
+
+
+# ---- the served-request benchmark ------------------------------------------------------------
+
+def request_stream(rate: float, duration: float, seed: int, max_cells: int = 20000,
+                   min_hours: int = 24, max_hours: int = 8760):
+    """Poisson arrivals at ``rate`` per second for ``duration`` s; cells log-uniform in
+    [1, max_cells], hours log-uniform in [min_hours, max_hours]; each its own seed."""
+    rng = np.random.default_rng(seed)
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return out
+        n = int(np.exp(rng.uniform(0.0, np.log(max_cells))))
+        h = int(np.exp(rng.uniform(np.log(min_hours), np.log(max_hours))))
+        out.append(Request(rid, max(n, 1), max(h, 1), seed=1_000_003 * (rid + 1) + seed,
+                           arrival=t))
+        rid += 1
+
+
+def serve(reqs, capacity: int, use_graph: bool = True, log=None):
+    """Run a request stream in real time: requests become admissible at their arrival time;
+    returns the completed requests with admission and completion times (s from start)."""
+    eng = ServeEngine(capacity, max_requests=max(64, min(len(reqs), 8192)), use_graph=use_graph)
+    queue = list(reqs)
+    waiting: list[Request] = []
+    done: list[Request] = []
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0
+    last = 0.0
+    while queue or waiting or eng.active or eng.retiring:
+        t = now()
+        while queue and queue[0].arrival <= t:
+            waiting.append(queue.pop(0))
+        while waiting and eng.admit(waiting[0], now()):
+            waiting.pop(0)
+        if eng.active:
+            eng.tick()
+        elif queue and not waiting:
+            time.sleep(max(0.0, min(queue[0].arrival - now(), 0.01)))
+        eng.retire_finished()
+        done += eng.collect(now, block=not eng.active and not queue and not waiting)
+        if log and now() - last > 5.0:
+            last = now()
+            log(f"t={last:6.1f}s active {len(eng.active)} waiting {len(waiting)} done {len(done)} "
+                f"ticks {eng.ticks}")
+    eng.sync()
+    return done, eng, now()
+
+
+def plain_per_request(reqs, budget_s: float):
+    """The same requests stepped one at a time with the plain step (one forcing, step and
+    accumulator launch sequence per request-hour), until ``budget_s`` seconds are spent."""
+    from .state import IBISState
+    from . import step as S
+    consts, sim = get_constants(), default_sim_params(bedrock=True)
+    froot = root_fractions()
+    cell_hours, t0 = 0, time.perf_counter()
+    for r in reqs:
+        f, slot, lat = synth.make_cells(r.ncell, r.seed)
+        st = IBISState.from_numpy(f, slot, synth.HSOI, froot, device="cuda")
+        acc = DR.Accumulators(st)
+        DR.run_hours(st, r.hours, DR.SyntheticForcing(lat, r.seed), consts, sim, acc=acc)
+        torch.cuda.synchronize()
+        cell_hours += r.ncell * r.hours
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return cell_hours / (time.perf_counter() - t0)
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="served-request benchmark of the hourly step")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--capacity", type=int, default=8 << 20,
+                    help="cells of resident capacity allocated (at most budget / bytes per cell)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain-budget", type=float, default=20.0)
+    args = ap.parse_args(argv)
+    cap = min(args.capacity, int(args.budget_gb * 1e9 // ServeEngine.bytes_per_cell()))
+    reqs = request_stream(args.rate, args.duration, args.seed)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log(f"{len(reqs)} requests, {sum(r.ncell * r.hours for r in reqs):.3e} cell-hours offered, "
+        f"capacity {cap} cells")
+    done, eng, wall = serve(reqs, cap, use_graph=not args.no_graph, log=log)
+    lat = np.array([r.done - r.arrival for r in done])
+    waitq = np.array([r.admitted - r.arrival for r in done])
+    ch = sum(r.ncell * r.hours for r in done)
+    plain = plain_per_request(reqs, args.plain_budget)
+    res = {"metric": "served cell-hours/s", "value": ch / wall, "unit": "cell-hours/s",
+           "requests": len(done), "wall_s": wall, "ticks": eng.ticks,
+           "latency_p50_s": float(np.percentile(lat, 50)),
+           "latency_p99_s": float(np.percentile(lat, 99)),
+           "admission_wait_p50_s": float(np.percentile(waitq, 50)),
+           "admission_wait_p99_s": float(np.percentile(waitq, 99)),
+           "plain_per_request_cell_hours_per_s": plain,
+           "speedup_vs_plain_per_request": (ch / wall) / plain,
+           "config": {"rate_per_s": args.rate, "duration_s": args.duration,
+                      "cells": "log-uniform 1-20000", "hours": "log-uniform 24-8760",
+                      "capacity_cells": cap, "budget_gb": args.budget_gb,
+                      "graph": not args.no_graph, "precision": "mixed"}}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

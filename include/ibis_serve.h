@@ -1,0 +1,62 @@
+/*
+ * C ABI of the served step: many independent runs ("requests") of the hourly step, each
+ * with its own cells, parameters, forcing seed and length, packed into one arena and
+ * stepped together, one launch sequence per hour for the whole batch.
+ *
+ * Every request's cells occupy a range of the packed arena that starts on a 64-cell
+ * boundary; a block of 64 cells (one wavefront) belongs to one request, named by the block
+ * table.  Each hour ("tick") runs, for every block whose request still has hours left:
+ * the benchmark forcing of the request's current hour (ibis_forcing_synthetic's generator,
+ * keyed by the request's seed, the cell's index within the request and the request's own
+ * hour), the step (timestep_body, python-port/ibis/timestepping/loops.py:234-1284) with the
+ * request's parameters, optionally the driver's per-step accumulators
+ * (loops_gswp3.py:161-302), and finally advances each active request's hour and nytimes.
+ * All per-hour state lives in device memory (request table, block table), so a tick can be
+ * captured once into a HIP graph and replayed; admission and retirement only rewrite table
+ * entries between ticks.
+ *
+ * No counterpart in the reference (its driver steps one grid per compiled scan).  Mixed
+ * precision only (FP32 state, FP64 solves).  Stream-ordered; nothing allocated.
+ */
+#ifndef IBIS_SERVE_H
+#define IBIS_SERVE_H
+
+#include <stdint.h>
+#include "ibis_driver.h"
+#include "ibis_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBIS_SERVE_BLOCK 64
+
+typedef struct IbisRequest {
+  IbisParams params;                   /* this request's parameter set */
+  double type_table[IBIS_TYPE_ROWS];   /* its plant-type Vmax table */
+  double dtime, tfac, zweight;         /* host constants of the step (ibis_step's) */
+  double nytimes;                      /* VegetationState.nytimes before the next hour */
+  int64_t start, ncell;                /* cell range in the packed arena; start % 64 == 0 */
+  int64_t hour;                        /* the request's own hour index of the next tick */
+  int64_t hours_left;                  /* ticks still to run; 0 = finished (blocks idle) */
+  uint64_t seed;                       /* forcing seed */
+  int32_t jday0;                       /* day of year of hour 0 */
+  int32_t bedrock;                     /* state carries soil.tsoi_deep */
+} IbisRequest;
+
+/* One hour of every active request.  arena / workspace / acc / vegx: packed planes of
+ * `pitch` (the arena's) cells; workspace of ibis_step_workspace_bytes(pitch, IBIS_PREC_MIXED)
+ * bytes; slot_type: IBIS_NPFT int8 planes of `pitch`; lat_deg: `pitch` doubles.
+ * block_req: nblock int32 request indices (-1 idle), nblock * 64 <= pitch; reqs: nreq
+ * records.  acc and vegx may be NULL (no accumulators).  Device pointers throughout. */
+int ibis_serve_tick(const IbisArena* arena, void* workspace, int64_t workspace_bytes,
+                    const int8_t* slot_type, const double* lat_deg, const IbisArena* acc,
+                    const IbisArena* vegx, const IbisAccParams* acc_params,
+                    IbisRequest* reqs, int64_t nreq, const int32_t* block_req, int64_t nblock,
+                    void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IBIS_SERVE_H */

@@ -28,7 +28,7 @@ def test_io_contract_matches_the_reference_leaf_count():
 def test_library_exports_every_declared_symbol():
     lib = _lib.load()
     decl = []
-    for h in ("ibis_step.h", "ibis_driver.h"):
+    for h in ("ibis_step.h", "ibis_driver.h", "ibis_serve.h"):
         with open(os.path.join(ROOT, "include", h)) as fh:
             decl += re.findall(r"^(?:int|int64_t|const char\*)\s+(ibis_\w+)\(", fh.read(), re.M)
     assert set(decl) == set(_lib.EXPORTS)
@@ -81,6 +81,7 @@ def test_workspace_size_per_precision():
 
 
 def test_product_defaults_equal_the_reference_defaults():
+    np.testing.assert_allclose(PP.root_fractions(), OP.root_fractions(), rtol=1e-15)
     c, oc = PP.get_constants(), OP.physical_constants()
     for k in vars(oc):
         assert getattr(c, k) == getattr(oc, k), k

@@ -1,0 +1,51 @@
+"""Served step, host side: the block allocator, request parameter draws and the request
+stream.  The GPU behaviour (alone vs in the mix, graph vs eager) is in test_gpu_serve.py."""
+import ctypes as C
+
+import numpy as np
+
+from ibis_amd import abi, serve as SV
+from ibis_amd.params import root_fractions
+
+
+def test_request_record_layout_is_the_c_struct():
+    # params, 52-row table, 4 doubles, 4 int64, uint64, 2 int32
+    assert C.sizeof(SV.IbisRequest) == C.sizeof(abi.IbisParams) + 8 * (52 + 4 + 4 + 1) + 8
+
+
+def test_parameter_draws_are_within_20_percent_and_reproducible():
+    froot = root_fractions()
+    base, t0 = SV.make_request_params(SV.Request(0, 10, 24, seed=5, perturb=False), froot)
+    p1, t1 = SV.make_request_params(SV.Request(0, 10, 24, seed=5), froot)
+    p2, t2 = SV.make_request_params(SV.Request(1, 10, 24, seed=5), froot)
+    np.testing.assert_array_equal(t1, t2)
+    assert bytes(p1) == bytes(p2)
+    for name in SV.PERTURBED:
+        a, b = np.atleast_1d(np.asarray(getattr(base, name))), np.atleast_1d(np.asarray(getattr(p1, name)))
+        r = b / np.where(a == 0, 1, a)
+        assert np.all((r >= 0.8) & (r <= 1.2)), name
+    assert np.all((t1 / t0 >= 0.8) & (t1 / t0 <= 1.2))
+    assert p1.alpha3 != base.alpha3
+
+
+def test_block_allocator_first_fit_and_merge():
+    eng = SV.ServeEngine.__new__(SV.ServeEngine)
+    eng.free_ranges = [(0, 100)]
+    a = eng._alloc(10); b = eng._alloc(20); c = eng._alloc(5)
+    assert (a, b, c) == (0, 10, 30)
+    eng._free(b, 20)
+    assert eng._alloc(25) == 35            # the 20-block hole is too small
+    assert eng._alloc(15) == 10            # first fit into the hole
+    eng._free(a, 10); eng._free(10, 15); eng._free(c, 5); eng._free(35, 25)
+    assert eng.free_ranges == [(0, 100)]   # every hole merged back
+
+
+def test_request_stream_distributions():
+    reqs = SV.request_stream(200.0, 20.0, seed=1)
+    n = np.array([r.ncell for r in reqs]); h = np.array([r.hours for r in reqs])
+    assert 3000 < len(reqs) < 5000
+    assert n.min() >= 1 and n.max() <= 20000 and h.min() >= 24 and h.max() <= 8760
+    # log-uniform: the median of log(cells) is near log(20000) / 2
+    assert abs(np.median(np.log(n)) - np.log(20000) / 2) < 0.5
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
