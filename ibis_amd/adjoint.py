@@ -1,0 +1,359 @@
+"""Gradients of hourly flux misfits with respect to per-pair tunable parameters.
+
+A *pair* is a site (one land cell with seeded state, slot types and benchmark forcing) and
+its own vector of the 42 tunable parameters of ``include/ibis_adjoint.h`` (quantum
+efficiencies, co-limitation curvatures, stomatal slopes and intercepts, stress factor, leaf
+and stem heat capacities, interception capacities, puddle and drainage limits, bedrock
+conductivity and heat capacity, and the 12 plant-type Vmax).  The loss of a pair is
+
+    L = sum over hours of sum over (GPP, LE, H) of ((flux - target) / scale)^2
+
+against target fluxes recorded from a run at other parameters.
+
+Derivatives are forward mode: the step's kernels instantiated in dual numbers carry, per
+thread, the derivative of the whole state along one parameter; a sweep over the window with
+42 x npair threads yields every dL/dtheta at once.  With 42 parameters against the ~75
+prognostic state words a reverse sweep would have to linearise, forward mode needs fewer
+step evaluations and no stored trajectory.  The window's states are checkpointed every
+``checkpoint_every`` hours by the value run; the tangent sweep reloads the value part of its
+state from each checkpoint as it passes it, so the derivative is taken along exactly the
+trajectory the value run computed (recomputation between checkpoints, never a stored tape).
+
+``FluxMisfit`` wraps the value run and the tangent sweep as a ``torch.autograd.Function``.
+Reference: the calibration studies of the reference take gradients with jax.grad through the
+scanned step (python-port/ibis/timestepping/loops.py:234-1284 under jax.lax.scan); the
+parameter defaults are those of python-port/ibis/physics/stomata.py:49-91, canopy_exchange.py:
+70-79, interception.py:33-45, timestepping/loops.py:92-94 and physics/bedrock.py:42-45.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import abi
+from . import layout as L
+from . import synth
+from .params import SnowConstants, default_sim_params, get_constants, root_fractions
+
+NTUNE = 42
+TUNE_VMAX = 30
+NFLUX = 3
+FLAG_TANGENT = 0x40
+FLAG_BEDROCK = 0x10
+PREC = {"mixed": abi.PREC_MIXED, "f64": abi.PREC_F64}
+
+#: tunable components in kernel order (IbisTune of ibis_dual.hpp)
+TUNE_NAMES = (["alpha3", "alpha4", "theta3", "theta4", "beta3", "beta4"]
+              + [f"coefm[{g}]" for g in range(5)] + [f"coefb[{g}]" for g in range(5)]
+              + ["stressfac", "cleaf", "cstem", "cgrass", "chu", "chs", "chl", "wliqumax",
+                 "wsnoumax", "fmax", "bperm", "wpudmax", "bed_consoi", "bed_csoi"]
+              + [f"vmax[{j}]" for j in range(L.NPFT)])
+assert len(TUNE_NAMES) == NTUNE
+
+#: flux scales of the misfit: GPP (mol CO2 m-2 s-1), LE and H (W m-2)
+FLUX_SCALE = (1.0e-5, 100.0, 100.0)
+
+
+class IbisTuned(C.Structure):
+    _fields_ = [("nsite", C.c_int64), ("npair", C.c_int64), ("ndir", C.c_int64),
+                ("dir0", C.c_int32), ("jday0", C.c_int32), ("seed", C.c_uint64),
+                ("hour", C.c_int64), ("dtime", C.c_double), ("nytimes", C.c_double),
+                ("flux_scale", C.c_double * NFLUX)]
+
+
+def _lib_tuned():
+    lib = _lib.load()
+    if not getattr(lib, "_tuned_typed", False):
+        lib.ibis_tuned_workspace_bytes.argtypes = [C.c_int64, C.c_int]
+        lib.ibis_tuned_workspace_bytes.restype = C.c_int64
+        lib.ibis_tuned_hour.argtypes = [C.POINTER(abi.IbisArena), C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(abi.IbisParams), C.POINTER(IbisTuned),
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        lib.ibis_tuned_hour.restype = C.c_int
+        lib._tuned_typed = True
+    return lib
+
+
+def base_params(co2=380.0e-6):
+    consts, sim = get_constants(), default_sim_params(bedrock=True)
+    return abi.params_from(consts, sim, SnowConstants(), synth.HSOI, root_fractions(), co2), sim
+
+
+def default_tune():
+    """The 42 tunable defaults, in kernel order."""
+    p, sim = base_params()
+    out = []
+    for name in TUNE_NAMES[:TUNE_VMAX]:
+        if "[" in name:
+            base, g = name[:-1].split("[")
+            out.append(float(getattr(p, base)[int(g)]))
+        else:
+            out.append(float(getattr(p, name)))
+    out += [float(v) for v in np.asarray(sim.stomata_params.vmax_pft, dtype=np.float64)]
+    return np.asarray(out, dtype=np.float64)
+
+
+def draw_tune(npair, seed, spread=0.2):
+    """(NTUNE, npair) parameter vectors uniform within +-spread of the defaults."""
+    d = default_tune()
+    u = np.stack([synth.uniform(seed, 700 + k, np.arange(npair, dtype=np.uint64), 0)
+                  for k in range(NTUNE)])
+    return d[:, None] * (1.0 - spread + 2.0 * spread * u)
+
+
+def row_pft():
+    """Plant-type position whose Vmax each type-table row takes (zone types: their parent)."""
+    return np.concatenate([synth.TYPE_PARENT, np.arange(L.NPFT)]).astype(np.int32)
+
+
+def _pitch(n):
+    return int(math.ceil(n / 256.0) * 256)
+
+
+@dataclass
+class Sites:
+    """Seeded sites: initial state (plane-major FP64), slot types, latitudes."""
+    nsite: int
+    seed: int = 3
+    jday0: int = 172
+
+    def __post_init__(self):
+        f, slot, lat = synth.make_cells(self.nsite, self.seed)
+        self.state64 = abi.pack(f, self.nsite, dtype=np.float64)      # (NPLANE, nsite)
+        self.slot = np.ascontiguousarray(np.asarray(slot, dtype=np.int8).T)  # (NPFT, nsite)
+        self.lat = np.asarray(lat, dtype=np.float64)
+
+
+class TunedRunner:
+    """Device buffers of one site set and the hourly launches over them."""
+
+    def __init__(self, sites: Sites, device="cuda", dtime=3600.0):
+        self.sites = sites
+        self.dev = torch.device(device)
+        self.dtime = float(dtime)
+        self.base, _ = base_params()
+        self.slot = torch.from_numpy(sites.slot).to(self.dev)
+        self.lat = torch.from_numpy(sites.lat).to(self.dev)
+        self.rows = torch.from_numpy(row_pft()).to(self.dev)
+        self.state64 = torch.from_numpy(sites.state64).to(self.dev)
+        self.stream = torch.cuda.current_stream(self.dev)
+
+    # -- state ------------------------------------------------------------------------------
+    def initial(self, npair, ndir=1, prec="mixed", tangent=False):
+        """Arena of ndir x npair threads at each pair's site's initial state (tangent zero)."""
+        n = self.sites.nsite
+        pitch = _pitch(ndir * npair)
+        dt = torch.float32 if prec == "mixed" else torch.float64
+        k = 2 if tangent else 1
+        a = torch.zeros((L.NPLANE, pitch, k), dtype=dt, device=self.dev)
+        src = self.state64.to(dt)                                         # (NPLANE, nsite)
+        reps = npair // n
+        a[:, :ndir * npair, 0] = src.repeat(1, reps * ndir)
+        return a
+
+    def workspace(self, pitch, prec, tangent):
+        lib = _lib_tuned()
+        flags = PREC[prec] | (FLAG_TANGENT if tangent else 0) | FLAG_BEDROCK
+        nb = int(lib.ibis_tuned_workspace_bytes(C.c_int64(pitch), C.c_int(flags)))
+        if nb < 0:
+            raise _lib.IbisError(f"ibis_tuned_workspace_bytes({pitch}, {flags}) = {nb}")
+        return torch.empty(nb, dtype=torch.uint8, device=self.dev)
+
+    # -- one hour ---------------------------------------------------------------------------
+    def hour(self, arena, ws, tune, hour, *, npair, ndir=1, dir0=-1, prec="mixed",
+             tangent=False, target=None, flux=None, loss=None, nytimes=None):
+        lib = _lib_tuned()
+        run = IbisTuned()
+        run.nsite, run.npair, run.ndir, run.dir0 = self.sites.nsite, npair, ndir, dir0
+        run.jday0, run.seed, run.hour = self.sites.jday0, self.sites.seed, hour
+        run.dtime = self.dtime
+        run.nytimes = float(hour if nytimes is None else nytimes)
+        run.flux_scale[:] = list(FLUX_SCALE)
+        flags = PREC[prec] | (FLAG_TANGENT if tangent else 0) | FLAG_BEDROCK
+        ar = abi.IbisArena(arena.data_ptr(), arena.shape[1])
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        rc = lib.ibis_tuned_hour(C.byref(ar), ptr(ws), C.c_int64(ws.numel()), ptr(self.slot),
+                                 ptr(self.lat), ptr(tune), ptr(self.rows), C.byref(self.base),
+                                 C.byref(run), ptr(target), ptr(flux), ptr(loss), C.c_int(flags),
+                                 C.c_void_p(self.stream.cuda_stream))
+        _lib.check(rc, "ibis_tuned_hour")
+
+    # -- runs -------------------------------------------------------------------------------
+    def record_fluxes(self, tune, hours, prec="mixed"):
+        """Hourly fluxes of a value run, (hours, NFLUX, npair) float32 on the device."""
+        npair = tune.shape[1]
+        a = self.initial(npair, prec=prec)
+        ws = self.workspace(a.shape[1], prec, False)
+        out = torch.zeros((hours, NFLUX, npair), dtype=torch.float32, device=self.dev)
+        fl = torch.zeros((NFLUX, a.shape[1]), dtype=torch.float32, device=self.dev)
+        for h in range(hours):
+            self.hour(a, ws, tune, h, npair=npair, prec=prec, flux=fl)
+            out[h].copy_(fl[:, :npair])
+        return out
+
+    def value(self, tune, target, prec="mixed", checkpoint_every=0):
+        """Loss per pair of a value run against target fluxes (hours, NFLUX, nsite); with
+        checkpoint_every > 0 also the states at hours 0, k, 2k, ... (before those hours)."""
+        npair = tune.shape[1]
+        hours = target.shape[0]
+        a = self.initial(npair, prec=prec)
+        ws = self.workspace(a.shape[1], prec, False)
+        loss = torch.zeros((2, a.shape[1]), dtype=torch.float64, device=self.dev)
+        ckpt = {}
+        for h in range(hours):
+            if checkpoint_every and h % checkpoint_every == 0:
+                ckpt[h] = a[:, :npair, 0].clone()
+            self.hour(a, ws, tune, h, npair=npair, prec=prec, target=target[h], loss=loss)
+        return loss[0, :npair].clone(), ckpt
+
+    def tangent(self, tune, target, prec="mixed", dirs=(0, NTUNE), checkpoints=None,
+                chunk=None):
+        """Loss per pair and dL/dtheta (NTUNE, npair) by a forward-mode sweep over the
+        components dirs[0] <= k < dirs[1] (others zero), ``chunk`` components per sweep."""
+        npair = tune.shape[1]
+        hours = target.shape[0]
+        d0, d1 = dirs
+        chunk = chunk or (d1 - d0)
+        grad = torch.zeros((NTUNE, npair), dtype=torch.float64, device=self.dev)
+        lval = None
+        for c0 in range(d0, d1, chunk):
+            nd = min(chunk, d1 - c0)
+            a = self.initial(npair, ndir=nd, prec=prec, tangent=True)
+            ws = self.workspace(a.shape[1], prec, True)
+            loss = torch.zeros((2, a.shape[1]), dtype=torch.float64, device=self.dev)
+            for h in range(hours):
+                if checkpoints is not None and h in checkpoints and h > 0:
+                    a[:, :nd * npair, 0] = checkpoints[h].repeat(1, nd)
+                self.hour(a, ws, tune, h, npair=npair, ndir=nd, dir0=c0, prec=prec,
+                          tangent=True, target=target[h], loss=loss)
+            grad[c0:c0 + nd] = loss[1, :nd * npair].view(nd, npair)
+            lval = loss[0, :npair].clone()
+        return lval, grad
+
+    def central_differences(self, tune, target, step=1.0e-3, prec="f64", dirs=(0, NTUNE)):
+        """dL/dtheta by central differences with relative step ``step`` on each component,
+        all variants of all pairs in one value run (npair x 2 x ncomponents threads)."""
+        npair = tune.shape[1]
+        d0, d1 = dirs
+        ks = list(range(d0, d1))
+        var = tune.repeat(1, 2 * len(ks))                  # (NTUNE, npair * 2 * nk)
+        hstep = torch.zeros(len(ks), npair, dtype=torch.float64, device=self.dev)
+        for j, k in enumerate(ks):
+            h = step * tune[k].abs()
+            hstep[j] = h
+            var[k, (2 * j) * npair:(2 * j + 1) * npair] += h
+            var[k, (2 * j + 1) * npair:(2 * j + 2) * npair] -= h
+        lv, _ = self.value(var.contiguous(), target, prec=prec)
+        lv = lv.view(len(ks), 2, npair)
+        grad = torch.zeros((NTUNE, npair), dtype=torch.float64, device=self.dev)
+        grad[d0:d1] = (lv[:, 0] - lv[:, 1]) / (2.0 * hstep)
+        return grad
+
+
+class FluxMisfit(torch.autograd.Function):
+    """loss (npair,) = misfit of the pairs' value run; backward: forward-mode tangent sweep
+    along every tunable component, restarted from the value run's checkpoints."""
+
+    @staticmethod
+    def forward(ctx, tune, runner: TunedRunner, target, prec="mixed", checkpoint_every=6):
+        loss, ckpt = runner.value(tune.detach(), target, prec=prec,
+                                  checkpoint_every=checkpoint_every)
+        ctx.runner, ctx.target, ctx.prec, ctx.ckpt = runner, target, prec, ckpt
+        ctx.save_for_backward(tune.detach())
+        return loss
+
+    @staticmethod
+    def backward(ctx, gout):
+        (tune,) = ctx.saved_tensors
+        _, grad = ctx.runner.tangent(tune, ctx.target, prec=ctx.prec, checkpoints=ctx.ckpt)
+        return grad * gout[None, :], None, None, None, None
+
+
+# ---- the parameter-gradient benchmark (4,096 pairs x 168 h) -----------------------------------
+
+def gradient_errors(g, g_ref, tune, floor=1e-3):
+    """Error statistics of g against g_ref (NTUNE, npair): relative error of the components
+    whose effect |g_ref * theta| is at least ``floor`` of the pair's largest, and the
+    per-pair error of the whole gradient in the pair's largest-effect units."""
+    g, g_ref, tune = (np.asarray(x, dtype=np.float64) for x in (g, g_ref, tune))
+    eff = np.abs(g_ref * tune)
+    scale = eff.max(axis=0)
+    live = scale > 0
+    sig = (eff >= floor * scale[None, :]) & live[None, :]
+    rel = np.abs(g - g_ref) / np.maximum(np.abs(g_ref), 1e-300)
+    pair = (np.abs(g - g_ref) * np.abs(tune)).max(axis=0)[live] / scale[live]
+    q = lambda x, p: float(np.quantile(x, p)) if x.size else float("nan")
+    return {"components": int(sig.sum()), "rel_median": q(rel[sig], 0.5),
+            "rel_p99": q(rel[sig], 0.99), "within_1e-3": float((rel[sig] <= 1e-3).mean()),
+            "pair_median": q(pair, 0.5), "pair_p99": q(pair, 0.99),
+            "pairs_within_1e-3": float((pair <= 1e-3).mean())}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    import time
+    ap = argparse.ArgumentParser(description="gradients of hourly flux misfits")
+    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--hours", type=int, default=168)
+    ap.add_argument("--checkpoint-every", type=int, default=6)
+    ap.add_argument("--fd-step", type=float, default=1e-3)
+    ap.add_argument("--no-fd", action="store_true")
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    sites = Sites(args.pairs, seed=3)
+    run = TunedRunner(sites)
+    dev = run.dev
+    tgt = torch.from_numpy(draw_tune(args.pairs, seed=91)).to(dev)
+    tune = torch.from_numpy(draw_tune(args.pairs, seed=17)).to(dev).requires_grad_(True)
+    sync = torch.cuda.synchronize
+    target = run.record_fluxes(tgt, args.hours)
+    sync()
+    log(f"target fluxes recorded: {args.pairs} pairs x {args.hours} h")
+    # warm the kernels once on a short window
+    FluxMisfit.apply(tune, run, target[:2], "mixed", args.checkpoint_every).sum().backward()
+    tune.grad = None
+    sync()
+    t0 = time.perf_counter()
+    loss = FluxMisfit.apply(tune, run, target, "mixed", args.checkpoint_every)
+    sync()
+    t1 = time.perf_counter()
+    loss.sum().backward()
+    sync()
+    t2 = time.perf_counter()
+    g = tune.grad.detach()
+    log(f"forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s")
+    res = {"metric": "gradient pair-hours/s", "unit": "pair-hours/s",
+           "value": args.pairs * args.hours / (t2 - t0),
+           "forward_s": t1 - t0, "backward_s": t2 - t1,
+           "forward_pair_hours_per_s": args.pairs * args.hours / (t1 - t0),
+           "loss_median": float(loss.detach().median()),
+           "config": {"pairs": args.pairs, "hours": args.hours, "tunables": NTUNE,
+                      "spread": 0.2, "checkpoint_every_h": args.checkpoint_every,
+                      "precision": "mixed forward and tangent", "mode": "forward (tangent)"}}
+    if not args.no_fd:
+        t3 = time.perf_counter()
+        g_fd = run.central_differences(tune.detach(), target, step=args.fd_step, prec="f64")
+        sync()
+        t4 = time.perf_counter()
+        lt64, g64 = run.tangent(tune.detach(), target, prec="f64", chunk=NTUNE // 2)
+        sync()
+        t5 = time.perf_counter()
+        th = tune.detach().cpu().numpy()
+        res["fd_f64"] = {"step": args.fd_step, "seconds": t4 - t3,
+                         "mixed_tangent_vs_fd": gradient_errors(g.cpu().numpy(), g_fd.cpu().numpy(), th),
+                         "f64_tangent_vs_fd": gradient_errors(g64.cpu().numpy(), g_fd.cpu().numpy(), th)}
+        res["f64_tangent_s"] = t5 - t4
+        log(json.dumps(res["fd_f64"]))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

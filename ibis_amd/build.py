@@ -14,6 +14,7 @@ host build's in the last bits.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -36,7 +37,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 # FP64 kernels, the C ABI and the driver keep IEEE division.
 FAST_DIV = ["-freciprocal-math", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 SOURCES = (("ibis_step.hip", []), ("ibis_step_fast.hip", FAST_DIV), ("ibis_step_f64.hip", []),
-           ("ibis_driver.hip", []), ("ibis_serve.hip", FAST_DIV))
+           ("ibis_driver.hip", []), ("ibis_serve.hip", FAST_DIV), ("ibis_tuned.hip", []),
+           ("ibis_tuned_fast.hip", FAST_DIV), ("ibis_tuned_f64.hip", []))
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
               "-Wno-unknown-pragmas", f"-I{INC}"]
 
@@ -63,11 +65,12 @@ def build(verbose: bool = False):
     out = os.path.join(HERE, "libibis_step.so")
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs, msg = [], ""
-    for name, extra in SOURCES:
-        obj = os.path.join(objdir, name.replace(".hip", ".o"))
-        msg += _run([hipcc, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, name), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(objdir, name.replace(".hip", ".o")) for name, _ in SOURCES]
+    jobs = [[hipcc, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, name), "-o", obj]
+            for (name, extra), obj in zip(SOURCES, objs)]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        msg = "".join(pool.map(_run, jobs))
     msg += _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     hout = os.path.join(HERE, "libibis_hostcheck.so")

@@ -1,0 +1,152 @@
+// Forward-mode derivatives of the hourly step: a dual number (value, one tangent) that the
+// precision-generic step code instantiates in place of float / double, and the parameter set
+// whose tunable members carry tangents.
+//
+// Every arithmetic operation computes its value exactly as the scalar code does (same
+// operation on the same operands), so the value part of a dual run follows the primal run;
+// the tangent part is the directional derivative along the seeded direction.  Comparisons
+// and branches read the value only: at a branch point the derivative is that of the branch
+// taken (the step's clamps and phase switches are piecewise smooth).
+#pragma once
+
+#include <type_traits>
+
+#include "ibis_math.hpp"
+#include "ibis_adjoint.h"
+#include "ibis_step.h"
+
+namespace ibis {
+
+template <class T> struct Dual {
+  T v, d;
+  IBIS_HD IBIS_INLINE Dual() = default;
+  IBIS_HD IBIS_INLINE constexpr Dual(T x, T dx) : v(x), d(dx) {}
+  template <class U, typename std::enable_if<std::is_arithmetic<U>::value, int>::type = 0>
+  IBIS_HD IBIS_INLINE constexpr Dual(U x) : v(T(x)), d(T(0)) {}
+  // widening (float -> double) converts implicitly, narrowing only on request, as for the
+  // underlying scalars in mixed expressions
+  template <class U, typename std::enable_if<(sizeof(U) < sizeof(T)), int>::type = 0>
+  IBIS_HD IBIS_INLINE constexpr Dual(const Dual<U>& o) : v(T(o.v)), d(T(o.d)) {}
+  template <class U, typename std::enable_if<(sizeof(U) > sizeof(T)), int>::type = 0>
+  IBIS_HD IBIS_INLINE explicit constexpr Dual(const Dual<U>& o) : v(T(o.v)), d(T(o.d)) {}
+  template <class U, typename std::enable_if<std::is_arithmetic<U>::value, int>::type = 0>
+  IBIS_HD IBIS_INLINE explicit operator U() const { return U(v); }
+
+  IBIS_HD IBIS_INLINE friend Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+  IBIS_HD IBIS_INLINE friend Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+  IBIS_HD IBIS_INLINE friend Dual operator-(Dual a) { return {-a.v, -a.d}; }
+  IBIS_HD IBIS_INLINE friend Dual operator*(Dual a, Dual b) {
+    return {a.v * b.v, a.d * b.v + a.v * b.d};
+  }
+  IBIS_HD IBIS_INLINE friend Dual operator/(Dual a, Dual b) {
+    const T q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+  }
+  IBIS_HD IBIS_INLINE Dual& operator+=(Dual b) { return *this = *this + b; }
+  IBIS_HD IBIS_INLINE Dual& operator-=(Dual b) { return *this = *this - b; }
+  IBIS_HD IBIS_INLINE Dual& operator*=(Dual b) { return *this = *this * b; }
+  IBIS_HD IBIS_INLINE Dual& operator/=(Dual b) { return *this = *this / b; }
+  IBIS_HD IBIS_INLINE friend bool operator<(Dual a, Dual b) { return a.v < b.v; }
+  IBIS_HD IBIS_INLINE friend bool operator>(Dual a, Dual b) { return a.v > b.v; }
+  IBIS_HD IBIS_INLINE friend bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+  IBIS_HD IBIS_INLINE friend bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+  IBIS_HD IBIS_INLINE friend bool operator==(Dual a, Dual b) { return a.v == b.v; }
+  IBIS_HD IBIS_INLINE friend bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+};
+
+template <class T> struct is_dual : std::false_type {};
+template <class T> struct is_dual<Dual<T>> : std::true_type {};
+
+// Elementary functions: value by the scalar routine, tangent by the chain rule.  A zero input
+// tangent gives a zero output tangent even where the derivative is infinite (sqrt at 0, pow
+// at a zero base), so an unseeded quantity never turns into NaN.
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_exp(Dual<T> a) {
+  const T e = r_exp(a.v);
+  return {e, e * a.d};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_log(Dual<T> a) {
+  return {r_log(a.v), a.d / a.v};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_sqrt(Dual<T> a) {
+  const T s = r_sqrt(a.v);
+  return {s, a.d == T(0) ? T(0) : a.d / (T(2) * s)};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_pow(Dual<T> a, Dual<T> b) {
+  const T p = r_pow(a.v, b.v);
+  T d = T(0);
+  if (a.d != T(0)) d = d + b.v * r_pow(a.v, b.v - T(1)) * a.d;
+  if (b.d != T(0)) d = d + p * r_log(a.v) * b.d;
+  return {p, d};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_abs(Dual<T> a) {
+  return {r_abs(a.v), a.v < T(0) ? -a.d : a.d};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_floor(Dual<T> a) { return {r_floor(a.v), T(0)}; }
+
+// max / min: the value is the scalar maximum / minimum (NaN-propagating, +0 above -0); the
+// tangent is the selected operand's (the first operand's on a tie).
+template <class T> IBIS_HD IBIS_INLINE Dual<T> mx(Dual<T> a, Dual<T> b) {
+  const T m = mx(a.v, b.v);
+  return {m, (m == a.v || b.v != b.v) ? a.d : b.d};
+}
+template <class T> IBIS_HD IBIS_INLINE Dual<T> mn(Dual<T> a, Dual<T> b) {
+  const T m = mn(a.v, b.v);
+  return {m, (m == a.v || b.v != b.v) ? a.d : b.d};
+}
+
+// ---- the tunable parameter vector ---------------------------------------------------------
+// The per-pair parameters a parameter-estimation run draws and differentiates against:
+// quantum efficiencies, co-limitation curvatures, stomatal slopes and intercepts, the
+// soil-water stress factor, leaf / stem heat capacities, interception capacities, puddle and
+// drainage limits and the bedrock thermal properties (python-port/ibis/physics/stomata.py:49-91,
+// canopy_exchange.py:70-79, interception.py:33-45, timestepping/loops.py:92-94,
+// physics/bedrock.py:42-45), then the 12 plant-type Vmax (stomata.py:84-88).
+enum IbisTune {
+  TUNE_ALPHA3 = 0, TUNE_ALPHA4, TUNE_THETA3, TUNE_THETA4, TUNE_BETA3, TUNE_BETA4,
+  TUNE_COEFM,                       // 5
+  TUNE_COEFB = TUNE_COEFM + 5,      // 5
+  TUNE_STRESSFAC = TUNE_COEFB + 5,
+  TUNE_CLEAF, TUNE_CSTEM, TUNE_CGRASS, TUNE_CHU, TUNE_CHS, TUNE_CHL,
+  TUNE_WLIQUMAX, TUNE_WSNOUMAX, TUNE_FMAX, TUNE_BPERM, TUNE_WPUDMAX,
+  TUNE_BED_CONSOI, TUNE_BED_CSOI,
+  TUNE_VMAX,                        // 12, per plant-type position
+  TUNE_N = TUNE_VMAX + IBIS_NPFT
+};
+static_assert(TUNE_N == IBIS_NTUNE, "tunable count");
+
+// The step's parameter set with the tunable members of type Q (double, or a dual number
+// seeded along one of them).  The members shadow the base set's; everything else is read
+// from the base.
+template <class Q> struct TunedParams : IbisParams {
+  Q alpha3, alpha4, theta3, theta4, beta3, beta4, coefm[5], coefb[5], stressfac;
+  Q cleaf, cstem, cgrass, chu, chs, chl, wliqumax, wsnoumax, fmax, bperm, wpudmax;
+  Q bed_consoi, bed_csoi;
+};
+
+// Build the tuned set of one pair: base values, the pair's tunable values tune[k * tpitch]
+// (k = IbisTune), and tangent 1 on component `dir` (dir < 0: no seed).
+template <class Q>
+IBIS_HD IBIS_INLINE void tune_params(TunedParams<Q>& t, const IbisParams& base,
+                                     const double* tune, int64_t tpitch, int dir) {
+  static_cast<IbisParams&>(t) = base;
+  auto q = [&](int k) -> Q {
+    const double v = tune[(int64_t)k * tpitch];
+    if constexpr (is_dual<Q>::value) return Q(v, k == dir ? 1.0 : 0.0);
+    else return Q(v);
+  };
+  t.alpha3 = q(TUNE_ALPHA3); t.alpha4 = q(TUNE_ALPHA4);
+  t.theta3 = q(TUNE_THETA3); t.theta4 = q(TUNE_THETA4);
+  t.beta3 = q(TUNE_BETA3); t.beta4 = q(TUNE_BETA4);
+  for (int g = 0; g < 5; ++g) {
+    t.coefm[g] = q(TUNE_COEFM + g);
+    t.coefb[g] = q(TUNE_COEFB + g);
+  }
+  t.stressfac = q(TUNE_STRESSFAC);
+  t.cleaf = q(TUNE_CLEAF); t.cstem = q(TUNE_CSTEM); t.cgrass = q(TUNE_CGRASS);
+  t.chu = q(TUNE_CHU); t.chs = q(TUNE_CHS); t.chl = q(TUNE_CHL);
+  t.wliqumax = q(TUNE_WLIQUMAX); t.wsnoumax = q(TUNE_WSNOUMAX); t.fmax = q(TUNE_FMAX);
+  t.bperm = q(TUNE_BPERM); t.wpudmax = q(TUNE_WPUDMAX);
+  t.bed_consoi = q(TUNE_BED_CONSOI); t.bed_csoi = q(TUNE_BED_CSOI);
+}
+
+}  // namespace ibis

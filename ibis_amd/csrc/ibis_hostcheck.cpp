@@ -144,3 +144,77 @@ extern "C" int ibis_hostcheck_accumulate(const void* state, int64_t pitch, const
                     apitch, ncell, *p);
   return 0;
 }
+
+// ---- parameter-estimation step (include/ibis_adjoint.h), host build of the same bodies ----
+#include "ibis_tuned_cell.hpp"
+
+namespace {
+
+template <class S, class R, class D, class Q>
+void tuned_host(S* a, const ibis::TunedArgs& g, const IbisParams& base,
+                const ibis::StepConst& K, const ibis::MisfitArgs& m, const float* target,
+                float* flux, double* loss) {
+  std::vector<unsigned char> ws((size_t)ibis::work_bytes(g.pitch, (int)sizeof(R),
+                                                         (int)sizeof(D)));
+  for (int64_t t = 0; t < g.nthread; ++t)
+    ibis::tuned_surface<S, R, D, Q>(a, ws.data(), g, ibis::tuned_thread_of(g, t), base, K);
+  for (int64_t t = 0; t < g.nthread; ++t)
+    ibis::tuned_canopy<S, R, D, Q>(a, ws.data(), g, ibis::tuned_thread_of(g, t), base, K);
+  for (int64_t t = 0; t < g.nthread; ++t)
+    ibis::tuned_ground<S, R, D, Q>(a, ws.data(), g, ibis::tuned_thread_of(g, t), base, K);
+  if (flux || (target && loss))
+    for (int64_t t = 0; t < g.nthread; ++t)
+      ibis::tuned_misfit<S>(a, g, ibis::tuned_thread_of(g, t), m, target, flux, loss);
+}
+
+}  // namespace
+
+extern "C" int ibis_hostcheck_tuned_hour(void* state, int64_t pitch, const int8_t* slot,
+                                         const double* lat, const double* tune,
+                                         const int32_t* row_pft, const IbisParams* base,
+                                         const IbisTuned* run, const float* target, float* flux,
+                                         double* loss, int flags) {
+  const bool tangent = (flags & IBIS_FLAG_TANGENT) != 0;
+  ibis::StepConst K;
+  K.dtime = run->dtime;
+  K.tfac = 1.0 / pow(0.999, base->cappa);
+  K.zweight = exp(-1.0 / (10.0 * 86400.0 / run->dtime));
+  K.nytimes_new = run->nytimes + 1.0;
+  K.bedrock = (flags & IBIS_FLAG_BEDROCK) ? 1 : 0;
+  ibis::TunedArgs g;
+  g.pitch = pitch;
+  g.nsite = run->nsite;
+  g.npair = run->npair;
+  g.nthread = run->ndir * run->npair;
+  g.dir0 = tangent ? run->dir0 : -1;
+  g.jday0 = run->jday0;
+  g.seed = run->seed;
+  g.hour = run->hour;
+  g.lat = lat;
+  g.tune = tune;
+  g.slot = slot;
+  g.row_pft = row_pft;
+  ibis::MisfitArgs m;
+  m.hvap = base->hvap;
+  for (int k = 0; k < IBIS_NFLUX; ++k) m.inv_scale[k] = 1.0 / run->flux_scale[k];
+  using ibis::Dual;
+  const int prec = flags & IBIS_PREC_MASK;
+  if (prec == IBIS_PREC_F64) {
+    if (tangent)
+      tuned_host<Dual<double>, Dual<double>, Dual<double>, Dual<double>>(
+          (Dual<double>*)state, g, *base, K, m, target, flux, loss);
+    else
+      tuned_host<double, double, double, double>((double*)state, g, *base, K, m, target, flux,
+                                                 loss);
+  } else if (prec == IBIS_PREC_MIXED) {
+    if (tangent)
+      tuned_host<Dual<float>, Dual<float>, Dual<double>, Dual<double>>(
+          (Dual<float>*)state, g, *base, K, m, target, flux, loss);
+    else
+      tuned_host<float, float, double, double>((float*)state, g, *base, K, m, target, flux,
+                                               loss);
+  } else {
+    return -2;
+  }
+  return 0;
+}

@@ -152,11 +152,11 @@ template <class R> struct Fwet { R fwetu, fwets, fwetl, rliqu, rliqs, rliql; };
 
 // compute_radiation, solar part: solsur + solalb + solarf per band (radiation.py:316-536,
 // 654-761); the infrared part is irrad below
-template <class R>
+template <class R, class PT>
 IBIS_HD void radiation(RadOut<R>& o, R coszen, const R solad[2], const R solai[2],
                        const R lai[2], const R sai[2], R fu, R fl, R tu, R ts, R tl,
                        R fi, R tsno1, R wsoi1, R wisoi1, R albsav, R albsan, const Fwet<R>& fw,
-                       const IbisParams& P) {
+                       const PT& P) {
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
   const bool sunlit = coszen > R(0);
   o.solu = o.sols = o.soll = o.solg = o.soli = R(0);
@@ -248,9 +248,9 @@ template <class D> struct IrOut { D firu, firs, firl, firg, firi, firb; };
 
 // irrad (radiation.py:543-619): net IR is a small difference of ~sigma T^4 terms, so the
 // solve precision D is used
-template <class D>
+template <class D, class PT>
 IBIS_HD IBIS_INLINE void irrad(IrOut<D>& o, D fira, const D lai[2], const D sai[2], D fu, D fl,
-                               D fi, D tu, D ts, D tl, D tg, D ti, const IbisParams& P) {
+                               D fi, D tu, D ts, D tl, D tg, D ti, const PT& P) {
   const D stef = D(P.stef);
   const D emu = D(1) - r_exp(-lai[1] / D(1));
   const D ems = D(1) - r_exp(-sai[1] / D(1));
@@ -290,8 +290,8 @@ IBIS_HD IBIS_INLINE void fwet_one(R t, R wl, R ws, R wlmax, R wsmax, R fmax, R t
   if (fwet == R(0)) rliq = (t < tmelt - R(1.0e-9)) ? R(0) : R(1);
 }
 
-template <class R>
-IBIS_HD IBIS_INLINE Fwet<R> fwetcal(R tu, R ts, R tl, const R w[6], const IbisParams& P) {
+template <class R, class PT>
+IBIS_HD IBIS_INLINE Fwet<R> fwetcal(R tu, R ts, R tl, const R w[6], const PT& P) {
   Fwet<R> o;
   const R fmax = R(P.fmax), tm = R(P.tmelt), eps = R(P.epsilon);
   fwet_one(tu, w[0], w[3], R(P.wliqumax), R(P.wsnoumax), fmax, tm, eps, o.fwetu, o.rliqu);
@@ -315,7 +315,7 @@ IBIS_HD IBIS_INLINE void mix3(R x1, R t1, R x2, R t2, R x3, R t3, R eps, R& xm, 
 
 // twet3 (interception.py:126-156): the latch freezes twk at the first converged step, so
 // the loop may stop there with the same result.
-template <class R> IBIS_HD IBIS_INLINE R twet3(R tak, R q, R p, const IbisParams& P) {
+template <class R, class PT> IBIS_HD IBIS_INLINE R twet3(R tak, R q, R p, const PT& P) {
   const R tmelt = R(P.tmelt);
   const R qs0 = qsat(esat(tak), p);
   const R ta = tak - tmelt;
@@ -337,10 +337,10 @@ template <class R> IBIS_HD IBIS_INLINE R twet3(R tak, R q, R p, const IbisParams
 template <class R> struct H2O { R wliq, wsno, pflux, rain, train, snow, tsnow; };
 
 // steph2o (interception.py:174-235)
-template <class R>
+template <class R, class PT>
 IBIS_HD IBIS_INLINE H2O<R> steph2o(R tveg, R wliq, R wsno, R xai, R rain, R train, R snow, R tsnow,
                                    R tdrip, R tblow, R wliqmax, R wsnomax, R wliqmin, R wsnomin,
-                                   R dtime, const IbisParams& P) {
+                                   R dtime, const PT& P) {
   H2O<R> o;
   const R eps = R(P.epsilon), tmelt = R(P.tmelt), ch2o = R(P.ch2o), cice = R(P.cice),
           hfus = R(P.hfus);
@@ -387,10 +387,10 @@ template <class R> struct Cascade {
 };
 
 // cascade (interception.py:263-357)
-template <class R>
+template <class R, class PT>
 IBIS_HD void cascade(Cascade<R>& o, R tu, R ts, R tl, R t12, R t34, const R w[6], const R lai[2],
                      const R sai[2], R fu, R fl, R ta, R qa, R psurf, R raina, R snowa, R dtime,
-                     const IbisParams& P) {
+                     const PT& P) {
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
   const R wliqmin = R(0.001) * (dtime / R(3600)) * (R(P.wliqumax) / R(0.2));
   const R wsnomin = R(0.001) * (dtime / R(3600)) * (R(P.wsnoumax) / R(2));
@@ -419,9 +419,9 @@ IBIS_HD void cascade(Cascade<R>& o, R tu, R ts, R tl, R t12, R t34, const R w[6]
 }
 
 // steph2o2 (interception.py:364-416)
-template <class R>
+template <class R, class PT>
 IBIS_HD IBIS_INLINE void steph2o2(R& tveg, R& wliq, R& wsno, R fveg, R xai, R rliq, R fvapw,
-                                  R cveg, R dtime, R ta, R& fvapa, R& fsena, const IbisParams& P) {
+                                  R cveg, R dtime, R ta, R& fvapa, R& fsena, const PT& P) {
   const R tmelt = R(P.tmelt), hfus = R(P.hfus), ch2o = R(P.ch2o), cice = R(P.cice);
   wliq = wliq - dtime * rliq * fvapw;
   wsno = wsno - dtime * (R(1) - rliq) * fvapw;
@@ -464,10 +464,10 @@ template <class R> struct Aero {
 };
 
 // canini (canopy_exchange.py:192-280); tfac is a host constant
-template <class R>
+template <class R, class PT>
 IBIS_HD void canini(Aero<R>& a, R ta, R qa, R psurf, R ua, R fi, R z0soi, R fl, R fu,
                     const R lai[2], const R sai[2], const R ztop[2], const R zbot[2],
-                    const IbisParams& P) {
+                    const PT& P) {
   const R siga = R(0.999);
   const R pa = psurf * siga;
   a.rhoa = pa / (R(P.rair) * ta * (R(1) + R(P.rvap / P.rair - 1.0) * qa));
@@ -556,10 +556,10 @@ IBIS_HD IBIS_INLINE void fstrat(R tb, R tt, R ttfac, R qb, R qt, R zb, R zt, R a
 template <class R> struct Turcof { R cu, cl, su, ss, sl, sg, si; };
 
 // turcof (canopy_exchange.py:333-426); u2/alogu/alogl/strat are carried between passes
-template <class R>
+template <class R, class PT>
 IBIS_HD void turcof(Turcof<R>& o, const Aero<R>& a, R tfac, R ta, R qa, R ua, R t12, R t34, R q12,
                     R q34, R dtime, bool first, R& u2c, R& aloguc, R& aloglc, R& straml,
-                    R& strahl, R& stramu, R& strahu, const IbisParams& P) {
+                    R& strahl, R& stramu, R& strahu, const PT& P) {
   const R vonk = R(P.vonk), grav = R(P.grav);
   fstrat(t34, t12, R(1), q34, q12, a.z3, a.z2, aloglc, aloglc, a.alog2, u2c, grav, vonk, first,
          straml, strahl);
@@ -612,8 +612,8 @@ IBIS_HD void turcof(Turcof<R>& o, const Aero<R>& a, R tfac, R ta, R qa, R ua, R 
 }
 
 // impexp (canopy_exchange.py:433-445); not called on the first pass (weight 1 there)
-template <class R>
-IBIS_HD IBIS_INLINE R impexp(R tveg, R ch, R wliq, R wsno, R prev, const IbisParams& P) {
+template <class R, class PT>
+IBIS_HD IBIS_INLINE R impexp(R tveg, R ch, R wliq, R wsno, R prev, const PT& P) {
   const R eps = R(P.epsilon), hfus = R(P.hfus);
   const R h = ch * (tveg - R(P.tmelt));
   const R z = mx(r_abs(h), eps);
@@ -625,7 +625,7 @@ IBIS_HD IBIS_INLINE R impexp(R tveg, R ch, R wliq, R wsno, R prev, const IbisPar
 }
 
 // impexp2 (canopy_exchange.py:448-463)
-template <class R> IBIS_HD IBIS_INLINE R impexp2(R t, R told, R prev, const IbisParams& P) {
+template <class R, class PT> IBIS_HD IBIS_INLINE R impexp2(R t, R told, R prev, const PT& P) {
   const R dt = t - told;
   const R wn = (dt > R(P.epsilon) + R(1.0e-8)) ? (R(P.tmelt) - told) / dt : prev;
   return clip(wn, R(0), R(1));

@@ -30,9 +30,9 @@ template <class R> IBIS_HD IBIS_INLINE R colimit(R j1, R j2, R theta) {
 template <class R> struct Leaf { R ag, an; };
 
 // farquhar_c3 (stomata.py:228-272)
-template <class R>
+template <class R, class PT>
 IBIS_HD IBIS_INLINE Leaf<R> farq_c3(R par, R ci, R vmax, R gstar, R kc, R ko, R o2, R rdark,
-                                    const IbisParams& P) {
+                                    const PT& P) {
   const R je = par * R(4.59e-06) * R(P.alpha3) * (ci - gstar) / (ci + R(2) * gstar);
   const R jc = vmax * (ci - gstar) / (ci + kc * (R(1) + o2 / ko));
   const R js = vmax / R(2.2);
@@ -43,8 +43,8 @@ IBIS_HD IBIS_INLINE Leaf<R> farq_c3(R par, R ci, R vmax, R gstar, R kc, R ko, R 
 }
 
 // farquhar_c4 (stomata.py:279-314)
-template <class R>
-IBIS_HD IBIS_INLINE Leaf<R> farq_c4(R par, R ci, R vmax, R rdark, const IbisParams& P) {
+template <class R, class PT>
+IBIS_HD IBIS_INLINE Leaf<R> farq_c4(R par, R ci, R vmax, R rdark, const PT& P) {
   const R je = par * R(4.59e-06) * R(P.alpha4);
   const R ji = R(18.0e3) * vmax * ci;
   Leaf<R> o;
@@ -90,7 +90,7 @@ IBIS_HD IBIS_INLINE R canopy_scaling(R toppar, R t6, R t7, const R sc[4], R lai,
 template <class R> struct TempDep { R kc, ko, gstar, tv3, tv4; };
 
 // _temp_dep_params (stomata.py:661-689)
-template <class R> IBIS_HD IBIS_INLINE TempDep<R> temp_dep(R tleaf, const IbisParams& P) {
+template <class R, class PT> IBIS_HD IBIS_INLINE TempDep<R> temp_dep(R tleaf, const PT& P) {
   TempDep<R> o;
   const R rw = R(3.47e-03) - R(1) / tleaf;
   const R tau = R(P.tau15) * r_exp(R(-4500) * rw);
@@ -120,11 +120,11 @@ template <class R> struct Stom {
 };
 
 // stomata (stomata.py:732-1080); vmax6 = (pft 0, 2, 4, 3, 5, 8, 10, 11) nominal values
-template <class R>
+template <class R, class PT>
 IBIS_HD void stomata(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R sl, R psurf,
                      const RadOut<R>& rad, R fwetu, R fwetl, const R lai[2], const R sai[2],
                      R vub, R vuc, R vls, R vl3, R vl4, R stresstu, R stresstl, R co2, R zw,
-                     const IbisParams& P) {
+                     const PT& P) {
   const R eps = R(P.epsilon), cimax = R(P.cimax), o2 = R(P.o2conc);
   // upper canopy
   TempDep<R> td = temp_dep(tu, P);
@@ -258,10 +258,10 @@ template <class D> struct TvOut {
 };
 
 // turvap (canopy_exchange.py:470-811)
-template <class R, class D>
+template <class R, class D, class PT>
 IBIS_HD void turvap(TvOut<D>& o, const TvFix<R, D>& f, const D cur[9], D cu, D cl, D su, D ss, D sl,
                     D sg, D si, const D tc[5], D wu, D ws, D wl, D wg, D wi, D dtime,
-                    const IbisParams& P) {
+                    const PT& P) {
   const D eps = D(P.epsilon), tmelt = D(P.tmelt);
   const D tu = wu * cur[0] + (D(1) - wu) * D(f.tuold);
   const D ts = ws * cur[1] + (D(1) - ws) * D(f.tsold);

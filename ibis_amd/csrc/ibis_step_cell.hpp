@@ -100,8 +100,8 @@ template <class R> IBIS_HD IBIS_INLINE void aero_load(Aero<R>& a, const R* v) {
 #define ST(P, v) c.st((P), S(v))
 #define LO(P) R(c.lo(P))
 
-template <class S, class R, class D>
-IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisParams& P,
+template <class S, class R, class D, class PT>
+IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
                           const StepConst& K) {
   IBIS_TRACE(c.i, "cell", 0);
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
@@ -312,9 +312,9 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisPara
   ST(IBIS_P_HYDROLOGY_TSNOWL, cas.tsnowl);
 }
 
-template <class S, class R, class D>
+template <class S, class R, class D, class PT>
 IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
-                         const IbisParams& P, const StepConst& K) {
+                         const PT& P, const StepConst& K) {
   const R eps = R(P.epsilon);
   const D dtD = D(K.dtime), tmeltD = D(P.tmelt);
 
@@ -543,8 +543,8 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
   wk.std_(WD_FVAPLT, tv.fvaplt);
 }
 
-template <class S, class R, class D>
-IBIS_HD void ground_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisParams& P,
+template <class S, class R, class D, class PT>
+IBIS_HD void ground_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
                          const StepConst& K) {
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
   const R dtime = R(K.dtime);
@@ -1163,9 +1163,9 @@ IBIS_HD void ground_cell(const Cell<S>& c, const Work<R, D>& wk, const IbisParam
 }
 
 // The three phases in order for one cell (host builds; the GPU runs them as three kernels).
-template <class S, class R, class D>
+template <class S, class R, class D, class PT>
 IBIS_HD void step_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
-                       const IbisParams& P, const StepConst& K) {
+                       const PT& P, const StepConst& K) {
   surface_cell<S, R, D>(c, wk, P, K);
   canopy_cell<S, R, D>(c, wk, vrow, P, K);
   ground_cell<S, R, D>(c, wk, P, K);

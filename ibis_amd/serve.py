@@ -16,7 +16,9 @@ not depend on what else is in the batch.
 """
 from __future__ import annotations
 
+import collections
 import ctypes as C
+import heapq
 import math
 import time
 from dataclasses import dataclass, field
@@ -79,7 +81,7 @@ class Request:
     start: int = -1
     admitted: float = float("nan")
     done: float = float("nan")
-    ticks_left: int = 0
+    finish_tick: int = 0           # engine tick after which the request is done
     result: dict = field(default_factory=dict)
     _copy: object = None
     _event: object = None
@@ -147,8 +149,14 @@ class ServeEngine:
         self.retiring: list[Request] = []
         self.use_graph = use_graph
         self.graphs = {}                                # launched blocks -> captured tick
+        # the host enqueues at most RUN_AHEAD ticks beyond the GPU, so that a request admitted
+        # now runs in one of the next few ticks and admission sees the GPU's real progress
+        self._tick_events = [torch.cuda.Event() for _ in range(self.RUN_AHEAD)]
         self.ticks = 0
         self.cell_hours = 0
+        self.active_cells = 0
+        self._finish = []                               # heap of (finish tick, rid, request)
+        self._top = 0                                   # one past the highest allocated block
 
     # -- memory -----------------------------------------------------------------------------
     @staticmethod
@@ -207,8 +215,12 @@ class ServeEngine:
             rb = torch.frombuffer(bytearray(bytes(rec)), dtype=torch.uint8)
             self.reqs[slot * self.recsize:(slot + 1) * self.recsize].copy_(rb.to(self.dev))
             self.blk[b0:b0 + nb].fill_(slot)
-        req.slot, req.start, req.ticks_left, req.admitted = slot, start, req.hours, now
+        req.slot, req.start, req.admitted = slot, start, now
+        req.finish_tick = self.ticks + req.hours
         self.active.append(req)
+        self.active_cells += n
+        heapq.heappush(self._finish, (req.finish_tick, req.rid, req))
+        self._top = max(self._top, b0 + nb)
         return True
 
     # -- ticks --------------------------------------------------------------------------------
@@ -217,10 +229,7 @@ class ServeEngine:
     def _grid(self):
         """Blocks to launch: up to the highest allocated block, rounded up to a bucket, so
         that a tick does not dispatch the idle tail of the capacity."""
-        top = 0
-        for r in self.active:
-            top = max(top, (r.start + r.ncell + BLOCK - 1) // BLOCK)
-        return min(self.nblock, -(-max(top, 1) // self.GRID_BUCKET) * self.GRID_BUCKET)
+        return min(self.nblock, -(-max(self._top, 1) // self.GRID_BUCKET) * self.GRID_BUCKET)
 
     def _launch(self, stream, nblock):
         lib = _lib_serve()
@@ -237,8 +246,14 @@ class ServeEngine:
                                  C.c_void_p(stream.cuda_stream))
         _lib.check(rc, "ibis_serve_tick")
 
+    RUN_AHEAD = 3
+
     def tick(self):
-        """One hour of every active request (enqueued, not waited for)."""
+        """One hour of every active request (enqueued; waits only for the tick RUN_AHEAD
+        ticks back)."""
+        ev = self._tick_events[self.ticks % self.RUN_AHEAD]
+        if self.ticks >= self.RUN_AHEAD:
+            ev.synchronize()
         nb = self._grid()
         if self.use_graph:
             g = self.graphs.get(nb)
@@ -251,20 +266,23 @@ class ServeEngine:
                 g.replay()
         else:
             self._launch(self.stream, nb)
+        ev.record(self.stream)
         self.ticks += 1
-        for r in self.active:
-            self.cell_hours += r.ncell
-            r.ticks_left -= 1
+        self.cell_hours += self.active_cells
 
     # -- retirement ---------------------------------------------------------------------------
     RESULT_KEYS = ("soil.tsoi", "soil.wsoi", "snow.fi", "canopy.tu")
 
     def retire_finished(self):
         """Start the result copies of requests whose hours are done and free their blocks."""
-        done = [r for r in self.active if r.ticks_left <= 0]
+        done = []
+        while self._finish and self._finish[0][0] <= self.ticks:
+            done.append(heapq.heappop(self._finish)[2])
         if not done:
             return
-        self.active = [r for r in self.active if r.ticks_left > 0]
+        gone = {r.rid for r in done}
+        self.active = [r for r in self.active if r.rid not in gone]
+        self.active_cells -= sum(r.ncell for r in done)
         with torch.cuda.stream(self.stream):
             for r in done:
                 nb = -(-r.ncell // BLOCK)
@@ -284,6 +302,7 @@ class ServeEngine:
                 self._free(s // BLOCK, nb)
                 self.free_slots.append(r.slot)
                 self.retiring.append(r)
+        self._top = max(((r.start + r.ncell + BLOCK - 1) // BLOCK for r in self.active), default=0)
 
     def collect(self, now_fn, block=False):
         """Requests whose result copies have landed; stamps their completion time."""
@@ -351,18 +370,22 @@ def serve(reqs, capacity: int, use_graph: bool = True, log=None):
     """Run a request stream in real time: requests become admissible at their arrival time;
     returns the completed requests with admission and completion times (s from start)."""
     eng = ServeEngine(capacity, max_requests=max(64, min(len(reqs), 8192)), use_graph=use_graph)
-    queue = list(reqs)
-    waiting: list[Request] = []
+    queue = collections.deque(reqs)
+    waiting: collections.deque[Request] = collections.deque()
     done: list[Request] = []
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0
     last = 0.0
+    window = reqs[-1].arrival if reqs else 0.0
+    eng.window_cell_hours = None
     while queue or waiting or eng.active or eng.retiring:
         t = now()
+        if eng.window_cell_hours is None and t >= window:
+            eng.window_cell_hours, eng.window_s = eng.cell_hours, t
         while queue and queue[0].arrival <= t:
-            waiting.append(queue.pop(0))
+            waiting.append(queue.popleft())
         while waiting and eng.admit(waiting[0], now()):
-            waiting.pop(0)
+            waiting.popleft()
         if eng.active:
             eng.tick()
         elif queue and not waiting:
@@ -427,6 +450,8 @@ def main(argv=None):
            "latency_p99_s": float(np.percentile(lat, 99)),
            "admission_wait_p50_s": float(np.percentile(waitq, 50)),
            "admission_wait_p99_s": float(np.percentile(waitq, 99)),
+           "arrival_window_cell_hours_per_s": (eng.window_cell_hours / eng.window_s
+                                               if eng.window_cell_hours else None),
            "plain_per_request_cell_hours_per_s": plain,
            "speedup_vs_plain_per_request": (ch / wall) / plain,
            "config": {"rate_per_s": args.rate, "duration_s": args.duration,

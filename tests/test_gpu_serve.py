@@ -51,3 +51,32 @@ def test_default_request_matches_the_plain_driver():
     rel = np.where(np.abs(ref) < 1e-20, np.abs(got - ref), rel)
     print("\nserved vs plain driver: max rel diff %.3e" % rel.max())
     assert rel.max() < 1e-5
+
+
+def test_requests_admitted_mid_run_into_freed_blocks_match_alone():
+    """Continuous batching: with a 4,096-cell arena, the third request does not fit until
+    the first retires; it is admitted into the freed blocks while the second is still
+    running, and every result equals the request stepped alone."""
+    reqs = [SV.Request(0, 3000, 20, seed=31), SV.Request(1, 700, 45, seed=32),
+            SV.Request(2, 2000, 24, seed=33), SV.Request(3, 5, 12, seed=34)]
+    arrive = {0: 0, 1: 3, 2: 5, 3: 30}
+    eng = SV.ServeEngine(4096, max_requests=8)
+    waiting, out, admitted_at, tick = [], {}, {}, 0
+    pending = sorted(reqs, key=lambda r: arrive[r.rid])
+    while pending or waiting or eng.active or eng.retiring:
+        while pending and arrive[pending[0].rid] <= tick:
+            waiting.append(pending.pop(0))
+        while waiting and eng.admit(waiting[0]):
+            admitted_at[waiting.pop(0).rid] = tick
+        if eng.active:
+            eng.tick()
+        tick += 1
+        eng.retire_finished()
+        for r in eng.collect(lambda: 0.0, block=not eng.active):
+            out[r.rid] = r.result["state_acc"]
+    assert admitted_at[2] == 20 and admitted_at[2] > arrive[2], admitted_at
+    assert reqs[2].start == 0          # the first request's blocks, reused
+    alone, _ = SV.run_alone_and_mixed(
+        [SV.Request(r.rid, r.ncell, r.hours, r.seed) for r in reqs], capacity=8192)
+    for rid, a in alone.items():
+        assert np.array_equal(a, out[rid]), rid

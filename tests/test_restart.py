@@ -1,0 +1,152 @@
+"""Restart files in the reference's layout: structure checks, the step's fields read from a
+real reference state, bit-exact round trips, and write-back after stepping.
+
+Fixture: ``tests/golden/site_restart_4cells.npz``, four cells of the reference's saved state
+``verify_out_stage1/site_transient_1996.npz`` in its own restart layout (made by
+``tests/golden/make_restart_fixture.py``)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from ibis_amd import restart as RS, layout as L
+from ibis_amd.params import SnowConstants
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIX = os.path.join(GOLD, "site_restart_4cells.npz")
+SITE = os.path.join(GOLD, "site_transient_1996_step.npz")
+CELLS = [0, 57, 114, 170]
+
+
+def _raw(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: np.array(z[k]) for k in z.files}
+
+
+def test_structure_of_a_reference_state():
+    r = RS.Restart.load(FIX)
+    assert len(r.leaves) == 258 and r.ncell == 4
+    assert [r.offset[g] for g in RS.SUBSTATES] == [0, 33, 47, 107, 129, 186, 210, 228]
+    assert r.trailing == [(None, 0)] * 4                 # no optional modules in this run
+    assert r.meta["year"] == 1996
+
+
+def test_step_fields_are_the_reference_leaves():
+    """The mapping by field order picks the leaves the site fixture (written by its own
+    script from the same file) holds, and the values are physically what they are named."""
+    r = RS.Restart.load(FIX)
+    f = r.fields()
+    site = _raw(SITE)
+    n = 0
+    for key, v in f.items():
+        if key in site:
+            np.testing.assert_array_equal(v, site[key][CELLS].astype(np.float64), err_msg=key)
+            n += 1
+    assert n >= 60
+    assert (f["soil.tsoi"] > 230).all() and (f["soil.tsoi"] < 320).all()
+    assert (f["soil.wsoi"] >= 0).all() and (f["soil.wsoi"] <= 1).all()
+    assert (f["soil.poros"] > 0.2).all() and (f["soil.poros"] < 0.7).all()
+    assert (f["snow.fi"] >= 0).all() and (f["snow.fi"] <= 1).all()
+    assert set(np.unique(f["vegetation.exist"])) <= {0.0, 1.0}
+    assert np.asarray(r.leaf("soil.hsoi")).shape == (7,)
+    assert np.asarray(r.leaf("vegetation.froot")).shape == (6, 2)
+
+
+def test_round_trip_is_bit_exact(tmp_path):
+    r = RS.Restart.load(FIX)
+    out = str(tmp_path / "rt.npz")
+    r.save(out)
+    a, b = _raw(FIX), _raw(out)
+    assert sorted(a) == sorted(b)
+    for k in a:
+        assert a[k].dtype == b[k].dtype and np.array_equal(a[k], b[k]), k
+
+
+def test_state_write_back(tmp_path):
+    r = RS.Restart.load(FIX)
+    st = r.to_state(device="cpu", dtype=torch.float32)
+    assert st.n == 4 and st.bedrock
+    np.testing.assert_array_equal(st.soil.tsoi.numpy(), np.asarray(r.leaf("soil.tsoi")))
+    assert st.snow_const == SnowConstants()
+    st.soil.tsoi = st.soil.tsoi + 1.5
+    st.canopy.tu = torch.full((4,), 290.25)
+    st.nytimes = 17.0
+    r.update(st)
+    out = str(tmp_path / "wb.npz")
+    r.save(out, metadata={"step": 17})
+    r2 = RS.Restart.load(out)
+    a, b = _raw(FIX), _raw(out)
+    changed = {r.index("soil.tsoi"), r.index("canopy.tu"), r.index("vegetation.nytimes")}
+    for i in range(258):
+        k = f"leaf_{i}"
+        assert a[k].dtype == b[k].dtype and a[k].shape == b[k].shape, k
+        if i not in changed:
+            assert np.array_equal(a[k], b[k]), k
+    np.testing.assert_array_equal(r2.leaf("soil.tsoi"),
+                                  (np.asarray(RS.Restart.load(FIX).leaf("soil.tsoi")) + np.float32(1.5)))
+    assert (r2.leaf("canopy.tu") == np.float32(290.25)).all()
+    assert r2.meta["step"] == 17 and r2.meta["year"] == 1996
+
+
+def _tree_with(trailing):
+    r = RS.Restart.load(FIX)
+    t = r.treedef_repr
+    tail = ", ".join(trailing)
+    return t.replace("None, None, None, None]", tail + "]"), r
+
+
+@pytest.mark.parametrize("mods,ok", [
+    ((), True),
+    ((("StressAccum", 4),), True),
+    ((("StressAccum", 3),), True),                       # written before the smoothed leaf
+    ((("StressAccum", 4), ("ETParts", 4), ("SeasonalState", 3), ("NEEParts", 2)), True),
+    ((("MethaneState", 3), ("StressAccum", 4)), True),     # wetland state before the chain
+    ((("StressAccum", 5),), False),
+    ((("StressAccum", 4), ("ETParts", 1)), False),
+])
+def test_optional_module_chain(mods, ok):
+    nodes = ["None"] * (8 - len(mods)) + [
+        f"CustomNode({m}[(Missing,)], [{', '.join(['*'] * c)}])" for m, c in mods]
+    tree, r = _tree_with(nodes)
+    extra = sum(c for _, c in mods)
+    leaves = r.leaves + [np.zeros(4, np.float32)] * extra
+    if ok:
+        rr = RS.Restart(leaves, tree, r.meta)
+        assert [c for _, c in rr.trailing if c] == [c for _, c in mods]
+        np.testing.assert_array_equal(rr.leaf("soil.tsoi"), r.leaf("soil.tsoi"))
+    else:
+        with pytest.raises(ValueError, match="optional-module chain"):
+            RS.Restart(leaves, tree, r.meta)
+
+
+def test_structure_mismatches_are_refused():
+    r = RS.Restart.load(FIX)
+    with pytest.raises(ValueError, match="leaves"):
+        RS.Restart(r.leaves[:-1], r.treedef_repr, r.meta)
+    bad = r.treedef_repr.replace("SnowState[(Missing,)], [*, ", "SnowState[(Missing,)], [", 1)
+    with pytest.raises(ValueError):
+        RS.Restart(r.leaves[:-1], bad, r.meta)
+    with pytest.raises(ValueError, match="root"):
+        RS.Restart([], "PyTreeDef(CustomNode(Other[(Missing,)], []))")
+
+
+def test_one_hour_from_a_reference_restart_on_the_host_build(tmp_path):
+    """Load a reference state, step it one hour with the host build of the kernel source,
+    write it back and read it again: the file holds the stepped fields."""
+    from _helpers import host_step
+    r = RS.Restart.load(FIX)
+    f = r.fields()
+    slot = np.full((4, L.NPFT), -1, dtype=np.int8)
+    out = host_step(f, slot, "mixed")
+    st = r.to_state(device="cpu")
+    for k in r.step_keys():
+        getattr(getattr(st, k.split(".")[0]), "__setattr__")(k.split(".")[1], torch.as_tensor(out[k]))
+    r.update(st)
+    p = str(tmp_path / "h1.npz")
+    r.save(p)
+    r2 = RS.Restart.load(p)
+    for k in ("soil.tsoi", "soil.wsoi", "canopy.tu", "snow.fi", "hydrology.gtrans"):
+        np.testing.assert_array_equal(np.asarray(r2.leaf(k), dtype=np.float64),
+                                      np.asarray(out[k], dtype=np.float32).astype(np.float64),
+                                      err_msg=k)
