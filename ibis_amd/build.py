@@ -38,7 +38,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 FAST_DIV = ["-freciprocal-math", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 SOURCES = (("ibis_step.hip", []), ("ibis_step_fast.hip", FAST_DIV), ("ibis_step_f64.hip", []),
            ("ibis_driver.hip", []), ("ibis_serve.hip", FAST_DIV), ("ibis_tuned.hip", []),
-           ("ibis_tuned_fast.hip", FAST_DIV), ("ibis_tuned_f64.hip", []))
+           ("ibis_tuned_fast.hip", FAST_DIV), ("ibis_tuned_f64.hip", []), ("ibis_daily.hip", []))
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
               "-Wno-unknown-pragmas", f"-I{INC}"]
 

@@ -218,3 +218,47 @@ extern "C" int ibis_hostcheck_tuned_hour(void* state, int64_t pitch, const int8_
   }
   return 0;
 }
+
+// ---- daily step (include/ibis_daily.h), host build of the same bodies ----------------------
+#include "ibis_daily_cell.hpp"
+
+namespace {
+
+template <class S>
+ibis::DayView<S> host_day_view(void* st, int64_t sp, void* day, int64_t dp, const void* daily,
+                               int64_t wp, void* acc, int64_t ap) {
+  ibis::DayView<S> v;
+  v.st = (S*)st;
+  v.day = (S*)day;
+  v.daily = (const S*)daily;
+  v.acc = (S*)acc;
+  v.sp = sp;
+  v.dp = dp;
+  v.wp = wp;
+  v.ap = ap;
+  v.i = 0;
+  return v;
+}
+
+}  // namespace
+
+extern "C" int ibis_hostcheck_day(int end, void* st, int64_t sp, void* day, int64_t dp,
+                                  const void* daily, int64_t wp, void* acc, int64_t ap,
+                                  void* vegx, int64_t vp, const IbisDayParams* p, int64_t ncell,
+                                  int flags) {
+  const bool f64 = (flags & IBIS_PREC_MASK) == IBIS_PREC_F64;
+  for (int64_t i = 0; i < ncell; ++i) {
+    if (f64) {
+      auto v = host_day_view<double>(st, sp, day, dp, daily, wp, acc, ap);
+      v.i = i;
+      if (end) ibis::day_end_cell(v, (double*)vegx, vp, *p);
+      else ibis::day_begin_cell(v, *p);
+    } else {
+      auto v = host_day_view<float>(st, sp, day, dp, daily, wp, acc, ap);
+      v.i = i;
+      if (end) ibis::day_end_cell(v, (float*)vegx, vp, *p);
+      else ibis::day_begin_cell(v, *p);
+    }
+  }
+  return 0;
+}

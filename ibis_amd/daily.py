@@ -1,0 +1,297 @@
+"""Daily step on the device: phenology at day begin, 10-day means, decomposition factors and
+soil biogeochemistry at day end (``include/ibis_daily.h``), and ``run_days``, the production
+driver's day loop (python-port/ibis/timestepping/loops_gswp3.py:382-583): per day, day begin,
+24 hours of diurnal forcing, step and accumulators, then day end -- every launch on one
+stream, nothing synchronised, so a month runs without host round trips.
+
+The day arena holds what the reference keeps in ``VegetationState`` (phenology triggers,
+growing degree-days, potential LAI per plant type, coldest-month temperature, root
+respiration, N mineralisation), ``CarbonState`` (all 24 fields, python-port/ibis/state/
+carbon.py) and ``SummaryState`` (10-day means and the annual sums the day end advances).
+Parameter defaults: ``PhenoParams`` (physics/phenology.py:20-85) and ``BGCParams``
+(io/parameters.py:176-250).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import abi
+from . import driver as DR
+from . import layout as L
+from . import step as S
+from . import synth
+from .state import IBISState
+
+CARBON = ("clitlm clitls clitll clitrm clitrs clitrl clitwm clitws clitwl csoipas csoislon "
+          "csoislop decompl decomps falll fallr fallw cdisturb totcmic totlit totalit totrlit "
+          "totcsoi totfall").split()
+#: day-arena fields in plane order (include/ibis_daily.h IbisDayPlane), with widths
+DAY_FIELDS = ([("tempu", 1), ("templ", 1), ("agddu", 1), ("agddl", 1), ("dropu", 1),
+               ("dropls", 1), ("dropl3", 1), ("dropl4", 1), ("plai", L.NPFT), ("tc", 1),
+               ("tco2root", 1), ("tnmin", 1)]
+              + [(c, 1) for c in CARBON]
+              + [("a10td", 1), ("a10ancub", 1), ("a10ancls", 1), ("a10ancl3", 1),
+                 ("a10ancl4", 1), ("ayprcp", 1), ("ayaet", 1), ("ayco2mic", 1),
+                 ("ayco2root", 1), ("storedn", 1), ("yrleach", 1), ("ynleach", 1)])
+DAY_OFFSET = {}
+_o = 0
+for _k, _w in DAY_FIELDS:
+    DAY_OFFSET[_k] = _o
+    _o += _w
+NDAYPLANE = _o
+assert NDAYPLANE == 59
+
+
+@dataclass
+class PhenoParams:
+    """physics/phenology.py:20-85 (thresholds; the seasonal-state switches are off)."""
+    tthresh_u_offset: float = 5.0
+    gthresh_u_c: float = 0.0
+    gdd_thresh_u: float = 100.0
+    gdd_ramp_u: float = 50.0
+    tthresh_l_c: float = 0.0
+    gthresh_l_c: float = -5.0
+    gdd_thresh_l: float = 150.0
+    gdd_ramp_l: float = 50.0
+    drop_thresh: float = 0.0
+    drop_floor: float = 0.1
+    ddays: float = 15.0
+    drop_stress_thresh: float = -1.0
+    tthresh_u_abs: float = -100.0
+
+
+@dataclass
+class BGCParams:
+    """io/parameters.py:176-250 (scalar Lloyd-Taylor E0; SOM-specific sensitivity off)."""
+    lig_frac: float = 0.50
+    fbsom: float = 0.017
+    effac: float = 0.40
+    fmax: float = 0.45
+    rconst: float = 1200.0
+    cnleaf: float = 40.0
+    cnroot: float = 60.0
+    cnwood: float = 200.0
+    cnr: tuple = (8.0, 15.0, 10.0, 15.0, 100.0, 150.0, 6.0, 250.0)
+    klm: float = 0.15
+    kls: float = 0.01
+    kll: float = 0.01
+    krm: float = 0.10
+    krs: float = 0.005
+    krl: float = 0.005
+    kwm: float = 0.001
+    kws: float = 0.001
+    kwl: float = 0.001
+    kbn: float = 0.045
+    kbp: float = 0.005
+    knb: float = 0.001
+    kns: float = 1.0e-6
+    kpb: float = 0.0001
+    kps: float = 1.0e-6
+    ksb: float = 8.0e-7
+    ylm: float = 0.4
+    yls: float = 0.3
+    yll: float = 1.0
+    yrm: float = 0.4
+    yrs: float = 0.3
+    yrl: float = 1.0
+    ywm: float = 0.4
+    yws: float = 0.3
+    ywl: float = 1.0
+    ybn: float = 1.0
+    ybp: float = 1.0
+    yps: float = 1.0
+    yns: float = 1.0
+    ysb: float = 0.20
+    ypb: float = 0.20
+    ynb: float = 0.25
+    lt_e0: float = 344.0
+    wfps_opt: float = 60.0
+    k_scale: float = 1.0
+
+
+_PH = [f.name for f in fields(PhenoParams)]
+_BG = [f.name for f in fields(BGCParams)]
+
+
+class IbisDayParams(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in _PH] + [("epsilon", C.c_double), ("tmelt", C.c_double)]
+                + [(n, C.c_double) for n in _BG[:8]] + [("cnr", C.c_double * 8)]
+                + [(n, C.c_double) for n in _BG[9:]]
+                + [("hsoi", C.c_double * (L.NSOILAY + 1)), ("ndaypy", C.c_double),
+                   ("day_sec", C.c_double), ("nsoilcall", C.c_int32), ("pad0", C.c_int32)])
+
+
+def day_params(hsoi, pheno: PhenoParams | None = None, bgc: BGCParams | None = None,
+               epsilon=1.0e-7, tmelt=273.16, nsoilcall=1, ndaypy=365.0, day_sec=86400.0):
+    pheno, bgc = pheno or PhenoParams(), bgc or BGCParams()
+    p = IbisDayParams()
+    for n in _PH:
+        setattr(p, n, float(getattr(pheno, n)))
+    for n in _BG:
+        if n == "cnr":
+            p.cnr[:] = [float(x) for x in bgc.cnr]
+        else:
+            setattr(p, n, float(getattr(bgc, n)))
+    p.epsilon, p.tmelt = float(epsilon), float(tmelt)
+    p.hsoi[:] = [float(x) for x in np.asarray(hsoi, dtype=np.float64)[:L.NSOILAY + 1]]
+    p.ndaypy, p.day_sec, p.nsoilcall = float(ndaypy), float(day_sec), int(nsoilcall)
+    return p
+
+
+def _lib_daily():
+    lib = _lib.load()
+    if not getattr(lib, "_daily_typed", False):
+        A = C.POINTER(abi.IbisArena)
+        lib.ibis_day_begin.argtypes = [A, A, A, A, C.POINTER(IbisDayParams), C.c_int64, C.c_int,
+                                       C.c_void_p]
+        lib.ibis_day_begin.restype = C.c_int
+        lib.ibis_day_end.argtypes = [A, A, A, A, A, C.POINTER(IbisDayParams), C.c_int64, C.c_int,
+                                     C.c_void_p]
+        lib.ibis_day_end.restype = C.c_int
+        lib._daily_typed = True
+    return lib
+
+
+class DayState:
+    """The day arena (NDAYPLANE, pitch) beside an hourly-step state."""
+
+    def __init__(self, state: IBISState, values: dict | None = None):
+        self.n, self.pitch = state.n, state.pitch
+        self.arena = torch.zeros((NDAYPLANE, state.pitch), dtype=state.dtype, device=state.device)
+        if values:
+            self.set(values)
+
+    def set(self, values: dict):
+        for k, v in values.items():
+            o, w = DAY_OFFSET[k], dict(DAY_FIELDS)[k]
+            t = torch.as_tensor(np.asarray(v, dtype=np.float64), device=self.arena.device)
+            if w == 1:
+                self.arena[o, :self.n] = t.reshape(self.n).to(self.arena.dtype)
+            else:
+                self.arena[o:o + w, :self.n] = t.reshape(self.n, w).t().to(self.arena.dtype)
+
+    def to_numpy(self) -> dict:
+        a = self.arena[:, :self.n].detach().cpu().numpy().astype(np.float64)
+        return {k: (a[DAY_OFFSET[k]] if w == 1 else a[DAY_OFFSET[k]:DAY_OFFSET[k] + w].T.copy())
+                for k, w in DAY_FIELDS}
+
+
+def _arena(t, pitch=None):
+    return abi.IbisArena(t.data_ptr(), t.shape[1] if pitch is None else pitch)
+
+
+def day_begin(state: IBISState, day: DayState, daily: torch.Tensor, acc: DR.Accumulators,
+              p: IbisDayParams, stream=None):
+    st = DR._stream(state, stream)
+    a, d, w, s = _arena(state.arena), _arena(day.arena), _arena(daily), _arena(acc.acc)
+    rc = _lib_daily().ibis_day_begin(C.byref(a), C.byref(d), C.byref(w), C.byref(s), C.byref(p),
+                                     C.c_int64(state.n), C.c_int(DR._code(state)),
+                                     C.c_void_p(st.cuda_stream))
+    _lib.check(rc, "ibis_day_begin")
+
+
+def day_end(state: IBISState, day: DayState, daily: torch.Tensor, acc: DR.Accumulators,
+            p: IbisDayParams, stream=None):
+    st = DR._stream(state, stream)
+    a, d, w, s = _arena(state.arena), _arena(day.arena), _arena(daily), _arena(acc.acc)
+    v = _arena(acc.vegx)
+    rc = _lib_daily().ibis_day_end(C.byref(a), C.byref(d), C.byref(w), C.byref(s), C.byref(v),
+                                   C.byref(p), C.c_int64(state.n), C.c_int(DR._code(state)),
+                                   C.c_void_p(st.cuda_stream))
+    _lib.check(rc, "ibis_day_end")
+
+
+# ---- seeded daily weather -------------------------------------------------------------------
+
+def precip_timing(precip, dtime, u_len, u_start):
+    """Length, start and end (s) of the day's precipitation event: at least 4 h, long
+    enough that no step exceeds 15 mm, placed uniformly inside the day
+    (python-port/ibis/physics/weather.py:395-441)."""
+    steps = int(86400.0 / dtime)
+    shortest = 1 + int((4.0 * 3600.0 - 1.0) / dtime)
+    longest = max(int(24.0 * 3600.0 / dtime), shortest)
+    n = shortest + np.floor(u_len * (longest - shortest + 1)).astype(np.int64)
+    n = np.maximum(n, np.ceil(np.asarray(precip) / 15.0).astype(np.int64))
+    n = np.clip(n, shortest, longest)
+    last = np.maximum(0, steps - n)
+    first = np.minimum(np.floor(u_start * (last + 1)).astype(np.int64), last)
+    return dtime * n, dtime * first, dtime * (first + n)
+
+
+class SyntheticWeather:
+    """Daily weather per cell and day from the counter-based generator: a seasonal cycle of
+    the daily mean temperature by latitude, diurnal range 4-16 K, relative humidity
+    0.3-0.9, wind 1-8 m/s, precipitation on 40 % of days (exponential, mean 6 mm/day),
+    cloud 0-1, the cell's surface pressure, and the event timing."""
+
+    def __init__(self, lat_deg, psurf, seed: int, jday0: int = 1, dtime: float = 3600.0):
+        self.lat = np.asarray(lat_deg, dtype=np.float64)
+        self.psurf = np.asarray(psurf, dtype=np.float64)
+        self.seed, self.jday0, self.dtime = int(seed), int(jday0), float(dtime)
+        self.n = len(self.lat)
+
+    def day(self, d: int) -> dict:
+        c = np.arange(self.n, dtype=np.uint64)
+        u = lambda k: synth.uniform(self.seed, 900 + k, c, d)
+        jday = self.jday0 + d
+        season = np.cos(2.0 * np.pi * (jday - 200) / 365.0) * np.sign(self.lat)
+        td = 288.0 - 0.35 * np.abs(self.lat) + 12.0 * season * np.abs(self.lat) / 60.0 \
+            + 4.0 * (u(0) - 0.5)
+        rng = 4.0 + 12.0 * u(1)
+        tmax, tmin = td + 0.5 * rng, td - 0.5 * rng
+        es = 611.0 * np.exp(17.27 * (td - 273.16) / (td - 35.86))
+        qd = (0.3 + 0.6 * u(2)) * 0.622 * es / (self.psurf - 0.378 * es)
+        wet = u(3) < 0.4
+        precip = np.where(wet, -6.0 * np.log(np.maximum(1.0 - u(4), 1e-12)), 0.0)
+        plens, startp, endp = precip_timing(precip, self.dtime, u(5), u(6))
+        return dict(td=td, tmax=tmax, tmin=tmin, qd=qd, ud=1.0 + 7.0 * u(7), precip=precip,
+                    cloud=u(8), psurf=self.psurf.copy(), plens=plens, startp=startp,
+                    endp=endp)
+
+
+def _daily_arena(w: dict, pitch, dtype, device):
+    a = np.zeros((len(DR.DAILY_NAMES), pitch), dtype=np.float64)
+    for j, k in enumerate(DR.DAILY_NAMES):
+        v = np.asarray(w[k], dtype=np.float64)
+        a[j, :len(v)] = v
+    return torch.as_tensor(a, device=device).to(dtype)
+
+
+def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR.Accumulators,
+             consts, params, days: int, p: IbisDayParams, dtime: float = 3600.0,
+             precision=None, day0: int = 0, stream=None, on_day=None):
+    """``days`` days of the production day loop: day begin, 24 hours (diurnal forcing from the
+    day's weather, step, accumulators), day end.  Weather for every day is generated and
+    copied before the loop, so the loop itself only enqueues launches."""
+    st = DR._stream(state, stream)
+    nstep = int(round(86400.0 / dtime))
+    wdays = [_daily_arena(weather.day(d), state.pitch, state.dtype, state.device)
+             for d in range(day0, day0 + days)]
+    cparams = S.ibis_params(state, consts, params)
+    lat = torch.as_tensor(weather.lat, device=state.device)
+    lib = DR._lib_driver()
+    a = _arena(state.arena)
+    for k, w in enumerate(wdays):
+        d = day0 + k
+        day_begin(state, day, w, acc, p, stream=st)
+        wa = _arena(w)
+        for h in range(nstep):
+            rc = lib.ibis_forcing_diurnal(C.byref(a), C.byref(wa), C.c_void_p(lat.data_ptr()),
+                                          C.c_int64(state.n), C.c_double(h * dtime),
+                                          C.c_int(weather.jday0 + d), C.c_double(dtime),
+                                          C.c_uint64(weather.seed), C.c_int64(0),
+                                          C.c_int64(d * nstep + h), C.c_int(DR._code(state)),
+                                          C.c_void_p(st.cuda_stream))
+            _lib.check(rc, "ibis_forcing_diurnal")
+            S.timestep_(state, consts, params, dtime, precision=precision, stream=st,
+                        cparams=cparams)
+            acc.add(state, stream=st)
+        day_end(state, day, w, acc, p, stream=st)
+        if on_day is not None:
+            on_day(d)
+    return state

@@ -1,0 +1,197 @@
+"""Daily step on the CPU: the kernels' source built for the host (libibis_hostcheck.so)
+against the oracle (oracle/daily.py), the C ABI's argument checks and the parameter
+struct's layout."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ibis_amd import abi, daily as DY, driver as DR, layout as L, synth
+from oracle import daily as OD
+from _helpers import HOSTCHECK
+from test_oracle_daily import HSOI, carbon_case, veg_case
+
+DAY_SUMS = ("adrain", "adsnow", "adaet", "a10ancub", "a10ancls", "a10ancl3", "a10ancl4")
+
+
+def day_case(n, seed=5):
+    """Hourly arena fields, day-arena values, daily weather and accumulators of n cells."""
+    rng = np.random.default_rng(seed)
+    f, slot, lat = synth.make_cells(n, seed)
+    f["soil.stresstu"] = rng.uniform(0, 1, n)
+    f["soil.stresstl"] = rng.uniform(0, 1, n)
+    v, a10 = veg_case(n, seed)
+    for k in ("frac", "lai", "ztop", "zbot", "fu", "fl"):
+        f["vegetation." + k] = v[k]
+    c, _, _, summ = carbon_case(n, seed)
+    day = dict(tempu=v["tempu"], templ=v["templ"], agddu=v["agddu"], agddl=v["agddl"],
+               dropu=v["dropu"], dropls=v["dropls"], dropl3=v["dropl3"], dropl4=v["dropl4"],
+               plai=v["plai"], tc=v["tc"], tco2root=rng.uniform(0, 1e-6, n), tnmin=np.zeros(n),
+               **c, **a10, ayprcp=rng.uniform(0, 500, n), ayaet=rng.uniform(0, 300, n),
+               ayco2mic=rng.uniform(0, 1, n), ayco2root=rng.uniform(0, 1, n), **summ)
+    td = rng.uniform(255, 305, n)
+    sums = {k: rng.uniform(0, 5, n) for k in DAY_SUMS[:3]}
+    sums.update({k: rng.uniform(-0.05, 0.5, n) for k in DAY_SUMS[3:]})
+    comp = {k: rng.uniform(-1e-7, 1e-7, n) for k in DAY_SUMS}
+    return f, day, td, sums, comp
+
+
+def _pack_day(day, n, dtype):
+    a = np.zeros((DY.NDAYPLANE, n), dtype=dtype)
+    for k, w in DY.DAY_FIELDS:
+        o = DY.DAY_OFFSET[k]
+        v = np.asarray(day[k], dtype=np.float64)
+        if w == 1:
+            a[o] = v
+        else:
+            a[o:o + w] = v.T
+    return a
+
+
+def _unpack_day(a):
+    return {k: (a[DY.DAY_OFFSET[k]].astype(np.float64) if w == 1
+                else a[DY.DAY_OFFSET[k]:DY.DAY_OFFSET[k] + w].T.astype(np.float64))
+            for k, w in DY.DAY_FIELDS}
+
+
+def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1):
+    n = len(td)
+    dt = np.float64 if prec == "f64" else np.float32
+    st = abi.pack(f, n, dtype=dt)
+    da = _pack_day(day, n, dt)
+    w = np.zeros((len(DR.DAILY_NAMES), n), dtype=dt)
+    w[0] = td
+    acc = np.zeros((2 * DR.NACC, n), dtype=dt)
+    for k in DAY_SUMS:
+        j = DR.ACC_NAMES.index(k)
+        acc[j] = sums[k] + comp[k]
+        acc[DR.NACC + j] = comp[k]
+    vx = np.zeros((DR.NVEGX, n), dtype=dt)
+    p = DY.day_params(HSOI, nsoilcall=nsoilcall)
+    ptr = lambda x: x.ctypes.data_as(C.c_void_p)
+    lib = C.CDLL(HOSTCHECK)
+    flags = abi.PREC_F64 if prec == "f64" else abi.PREC_MIXED
+    rc = lib.ibis_hostcheck_day(C.c_int(end), ptr(st), C.c_int64(n), ptr(da), C.c_int64(n),
+                                ptr(w), C.c_int64(n), ptr(acc), C.c_int64(n), ptr(vx),
+                                C.c_int64(n), C.byref(p), C.c_int64(n), C.c_int(flags))
+    assert rc == 0
+    return abi.unpack(st, n), _unpack_day(da), acc, vx
+
+
+def oracle_begin(f, day, td):
+    v = {k: day[k] for k in ("tempu", "templ", "agddu", "agddl", "dropu", "dropls", "dropl3",
+                             "dropl4", "plai", "tc")}
+    for k in ("frac", "lai", "ztop", "zbot", "fu", "fl"):
+        v[k] = np.asarray(f["vegetation." + k], dtype=np.float64)
+    a10 = {k: day[k] for k in ("a10td", "a10ancub", "a10ancls", "a10ancl3", "a10ancl4")}
+    return OD.pheno(v, a10, td, 1.0e-7, OD.PhenoParams(), f["soil.stresstu"], f["soil.stresstl"])
+
+
+def _round(d, dt):
+    return {k: np.asarray(v, dtype=dt).astype(np.float64) if isinstance(v, np.ndarray) else v
+            for k, v in d.items()}
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_host_day_begin_matches_the_oracle(prec):
+    n = 3000
+    f, day, td, sums, comp = day_case(n)
+    dt = np.float64 if prec == "f64" else np.float32
+    if prec == "mixed":                          # the oracle sees the FP32-rounded inputs
+        f, day, td = _round(f, dt), _round(day, dt), np.asarray(td, dt).astype(np.float64)
+    st, da, acc, _ = host_day(0, f, day, td, sums, comp, prec)
+    ref = oracle_begin(f, day, td)
+    # FP64: relative to each value; FP32: relative to the field's magnitude over the cells
+    # (agdd - threshold and its ramp cancel, leaving a few FP32 ulps of the field's scale)
+    def check(got, want, k):
+        if prec == "f64":
+            np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-300, err_msg=k)
+        else:
+            err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-30)
+            assert err <= 1e-5, (k, err)
+    for k in ("tempu", "templ", "agddu", "agddl", "dropu", "dropls", "dropl3", "dropl4"):
+        check(da[k], ref[k], k)
+    for k in ("frac", "lai", "ztop", "zbot"):
+        check(st["vegetation." + k], ref[k], k)
+    for k in DAY_SUMS:                            # the day's sums start from zero
+        j = DR.ACC_NAMES.index(k)
+        assert (acc[j] == 0).all() and (acc[DR.NACC + j] == 0).all()
+    # untouched accumulators keep their values
+    j = DR.ACC_NAMES.index("aygpptot")
+    assert (acc[j] == 0).all()
+
+
+@pytest.mark.parametrize("prec,nsoilcall", [("f64", 1), ("f64", 3), ("mixed", 1)])
+def test_host_day_end_matches_the_oracle(prec, nsoilcall):
+    n = 3000
+    f, day, td, sums, comp = day_case(n, seed=8)
+    dt = np.float64 if prec == "f64" else np.float32
+    if prec == "mixed":
+        f, day, td = _round(f, dt), _round(day, dt), np.asarray(td, dt).astype(np.float64)
+        sums = {k: np.asarray(v + comp[k], dt).astype(np.float64) - np.asarray(comp[k], dt).astype(np.float64)
+                for k, v in sums.items()}
+    st, da, acc, vx = host_day(1, f, day, td, sums, comp, prec, nsoilcall)
+    ref, tco2mic, tnmin = OD.day_end(day, sums, td, f["soil.tsoi"], f["soil.wsoi"],
+                                     f["soil.wisoi"], f["soil.sand"], f["soil.clay"], HSOI,
+                                     OD.BGCParams(), nsoilcall=nsoilcall)
+    # FP64: relative to each value (floor 1e-6 of the field's largest); FP32: relative to the
+    # field's magnitude over the cells (10-day means of mixed-sign assimilation cancel)
+    tol = 1e-12 if prec == "f64" else 1e-5
+    for k, w in DY.DAY_FIELDS:
+        if k == "tnmin":
+            continue
+        floor = 1e-6 if prec == "f64" else 1.0
+        scale = np.maximum(np.abs(ref[k]), floor * np.abs(ref[k]).max() + 1e-300)
+        err = np.abs(da[k] - ref[k]) / scale
+        assert err.max() <= tol, (k, err.max())
+    np.testing.assert_allclose(da["tnmin"], tnmin, rtol=tol, atol=1e-12)
+    # tco2mic is the day's carbon balance, (pools before + input - pools after - leaching) /
+    # 1036.8: pools of tens of kg C against a flux of 1e-4, so FP32 keeps ~1e-5 of its scale
+    tco = vx[L_TCO2MIC].astype(np.float64)
+    if prec == "f64":
+        np.testing.assert_allclose(tco, tco2mic, rtol=1e-9, atol=1e-15)
+    else:
+        assert np.abs(tco - tco2mic).max() / np.abs(tco2mic).max() < 1e-4
+
+
+L_TCO2MIC = 25
+
+
+def test_day_params_struct_layout():
+    assert C.sizeof(DY.IbisDayParams) == 75 * 8 + 8
+    p = DY.day_params(HSOI)
+    assert p.cnr[4] == 100.0 and p.klm == 0.15 and p.ynb == 0.25 and p.k_scale == 1.0
+    assert p.hsoi[6] == 4.0 and p.nsoilcall == 1 and p.ddays == 15.0
+    assert DR.NVEGX == 26 and L_TCO2MIC == 25
+
+
+def test_day_entry_points_reject_bad_arguments():
+    lib = DY._lib_daily()
+    fake = abi.IbisArena(C.c_void_p(256), 100)
+    small = abi.IbisArena(C.c_void_p(256), 10)
+    p = DY.day_params(HSOI)
+    A = C.byref
+    assert lib.ibis_day_begin(A(fake), A(small), A(fake), A(fake), A(p), 100, abi.PREC_MIXED, None) == -1
+    assert lib.ibis_day_begin(A(fake), A(fake), A(fake), A(fake), A(p), 100, 0x100, None) == -2
+    assert lib.ibis_day_begin(None, A(fake), A(fake), A(fake), A(p), 100, 0, None) == -1
+    bad = DY.day_params(HSOI)
+    bad.ddays = 0.0
+    assert lib.ibis_day_begin(A(fake), A(fake), A(fake), A(fake), A(bad), 100, 0, None) == -1
+    assert lib.ibis_day_end(A(fake), A(fake), A(fake), A(fake), None, A(p), 100, 0, None) == -1
+    assert lib.ibis_day_end(A(fake), A(fake), A(fake), A(fake), A(small), A(p), 100, 0, None) == -1
+    assert b"vegetation extras" in lib.ibis_last_error()
+    assert lib.ibis_day_end(A(fake), A(fake), A(fake), A(fake), A(fake), A(p), 0, 0, None) == 0
+
+
+def test_product_precip_timing_matches_the_oracle():
+    from oracle import driver as ODR
+    rng = np.random.default_rng(4)
+    n = 5000
+    precip = np.where(rng.random(n) < 0.5, rng.exponential(20.0, n), 0.0)
+    precip[:5] = [0.0, 15.0, 15.01, 400.0, 1000.0]
+    ul, us = rng.random(n), rng.random(n)
+    for dtime in (3600.0, 1800.0):
+        got = DY.precip_timing(precip, dtime, ul, us)
+        ref = ODR.precip_timing(precip, dtime, ul, us)
+        for g, r in zip(got, ref):
+            np.testing.assert_array_equal(g, r)

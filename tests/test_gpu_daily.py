@@ -1,0 +1,108 @@
+"""Daily step on the GPU: the day-begin and day-end kernels against the host build of the
+same source, and a few days of the day loop (day begin, 24 driven hours, day end)."""
+import numpy as np
+import pytest
+import torch
+
+from ibis_amd import abi, daily as DY, driver as DR, synth
+from ibis_amd.params import default_sim_params, get_constants, root_fractions
+from ibis_amd.state import IBISState
+from oracle import daily as OD
+from test_daily import DAY_SUMS, L_TCO2MIC, day_case, host_day
+from test_oracle_daily import HSOI
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_day(end, f, day, td, sums, comp, prec, nsoilcall=1):
+    n = len(td)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    st = IBISState.from_numpy(f, None, HSOI, root_fractions(), device="cuda", dtype=dt)
+    ds = DY.DayState(st, day)
+    w = torch.zeros((len(DR.DAILY_NAMES), n), dtype=dt, device="cuda")
+    w[0] = torch.as_tensor(td, device="cuda").to(dt)
+    acc = DR.Accumulators(st)
+    for k in DAY_SUMS:
+        j = DR.ACC_NAMES.index(k)
+        acc.acc[j] = torch.as_tensor(sums[k] + comp[k], device="cuda").to(dt)
+        acc.acc[DR.NACC + j] = torch.as_tensor(comp[k], device="cuda").to(dt)
+    p = DY.day_params(HSOI, nsoilcall=nsoilcall)
+    (DY.day_end if end else DY.day_begin)(st, ds, w, acc, p)
+    torch.cuda.synchronize()
+    return (st.to_numpy(), ds.to_numpy(), acc.acc.cpu().numpy().astype(np.float64),
+            acc.vegx.cpu().numpy().astype(np.float64))
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_day_begin_kernel_is_the_host_build(prec):
+    """Phenology has no transcendental: the GPU and host builds agree bit for bit."""
+    f, day, td, sums, comp = day_case(4000, seed=11)
+    st, da, acc, _ = gpu_day(0, f, day, td, sums, comp, prec)
+    hst, hda, hacc, _ = host_day(0, f, day, td, sums, comp, prec)
+    for k in ("tempu", "templ", "agddu", "agddl", "dropu", "dropls", "dropl3", "dropl4"):
+        np.testing.assert_array_equal(da[k], hda[k], err_msg=k)
+    for k in ("frac", "lai", "ztop", "zbot"):
+        np.testing.assert_array_equal(st["vegetation." + k], hst["vegetation." + k], err_msg=k)
+    np.testing.assert_array_equal(acc, hacc)
+
+
+@pytest.mark.parametrize("prec,nsoilcall", [("f64", 1), ("f64", 2), ("mixed", 1)])
+def test_day_end_kernel_matches_the_host_build_and_the_oracle(prec, nsoilcall):
+    f, day, td, sums, comp = day_case(4000, seed=12)
+    st, da, acc, vx = gpu_day(1, f, day, td, sums, comp, prec, nsoilcall)
+    hst, hda, hacc, hvx = host_day(1, f, day, td, sums, comp, prec, nsoilcall)
+    tol = 1e-12 if prec == "f64" else 1e-5       # device exp vs the host's libm
+    for k, _ in DY.DAY_FIELDS:
+        scale = np.maximum(np.abs(hda[k]), 1e-6 * np.abs(hda[k]).max() + 1e-300)
+        err = (np.abs(da[k] - hda[k]) / scale).max()
+        assert err <= tol, (k, err)
+    t, ht = vx[L_TCO2MIC], hvx[L_TCO2MIC]
+    assert np.abs(t - ht).max() <= (1e-9 if prec == "f64" else 1e-4) * np.abs(ht).max()
+    if prec == "f64":
+        ref, tco2mic, _ = OD.day_end(day, sums, td, f["soil.tsoi"], f["soil.wsoi"],
+                                     f["soil.wisoi"], f["soil.sand"], f["soil.clay"], HSOI,
+                                     OD.BGCParams(), nsoilcall=nsoilcall)
+        for k in OD.CARBON + ["a10td", "a10ancub", "ayprcp", "storedn", "ayco2mic"]:
+            scale = np.maximum(np.abs(ref[k]), 1e-6 * np.abs(ref[k]).max() + 1e-300)
+            assert (np.abs(da[k] - ref[k]) / scale).max() <= 1e-12, k
+
+
+def test_three_days_of_the_day_loop():
+    """Day begin, 24 hours of diurnal forcing, step and accumulators, day end, three times:
+    the phenology and carbon states move, every field stays finite, the day's GPP sum is
+    folded into nothing it should not be, and the mixed run stays within 1e-3 of an FP64
+    run of the same days."""
+    n = 2048
+    f, slot, lat = synth.make_cells(n, 21)
+    out = {}
+    for prec, dt in (("f64", torch.float64), ("mixed", torch.float32)):
+        st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda", dtype=dt)
+        rng = np.random.default_rng(3)
+        vals = dict(tempu=np.full(n, 0.5), templ=np.full(n, 0.5), agddu=np.full(n, 80.0),
+                    agddl=np.full(n, 120.0), dropu=np.ones(n), dropls=np.ones(n),
+                    dropl3=np.ones(n), dropl4=np.ones(n), plai=rng.uniform(0.5, 5, (n, 12)),
+                    tc=rng.uniform(-20, 15, n), a10td=np.asarray(f["soil.tsoi"])[:, 0],
+                    clitlm=np.full(n, 0.1), clitls=np.full(n, 0.05), clitrm=np.full(n, 0.08),
+                    clitrs=np.full(n, 0.04), csoipas=np.full(n, 8.0), csoislon=np.full(n, 2.0),
+                    csoislop=np.full(n, 2.0), totcmic=np.full(n, 0.1),
+                    falll=np.full(n, 0.3), fallr=np.full(n, 0.3), fallw=np.full(n, 0.2))
+        day = DY.DayState(st, vals)
+        acc = DR.Accumulators(st)
+        w = DY.SyntheticWeather(lat, np.asarray(f["atm.psurf"]), seed=5, jday0=150)
+        p = DY.day_params(synth.HSOI)
+        DY.run_days(st, day, w, acc, get_constants(), default_sim_params(), 3, p)
+        torch.cuda.synchronize()
+        out[prec] = (st.to_numpy(), day.to_numpy(), acc.values())
+    s64, d64, a64 = out["f64"]
+    s32, d32, a32 = out["mixed"]
+    for d in (d64, d32):
+        for k, v in d.items():
+            assert np.isfinite(v).all(), k
+    assert not np.allclose(d64["agddu"], 80.0)                     # phenology moved
+    assert (d64["ayco2mic"] > 0).all() and (d64["ayprcp"] >= 0).all()
+    np.testing.assert_allclose(s64["vegetation.frac"][:, :8].sum(1), 1.0, rtol=1e-12)
+    for k in ("agddu", "tempu", "csoipas", "totcmic", "ayco2mic", "a10td", "ayaet"):
+        scale = np.abs(d64[k]).max()
+        err = np.abs(d32[k] - d64[k]).max() / max(scale, 1e-30)
+        print(f"\n{k}: mixed vs FP64 after 3 days, max err / field scale {err:.2e}")
+        assert err < 1e-3, k
