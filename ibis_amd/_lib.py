@@ -12,10 +12,11 @@ from .abi import IbisArena, IbisParams
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libibis_step.so")
 
-#: every entry point declared in include/ibis_step.h, ibis_driver.h and ibis_serve.h
+#: every entry point declared in include/*.h
 EXPORTS = ("ibis_step", "ibis_step_workspace_bytes", "ibis_last_error", "ibis_version",
            "ibis_forcing_synthetic", "ibis_forcing_diurnal", "ibis_accumulate",
-           "ibis_serve_tick")
+           "ibis_serve_tick", "ibis_tuned_workspace_bytes", "ibis_tuned_hour",
+           "ibis_day_begin", "ibis_day_end")
 
 _lib = None
 

@@ -5,12 +5,28 @@
 // loads it.  Same template instantiations, same operation order, no FMA contraction.
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include <vector>
 
+#include "ibis_dual.hpp"
 #include "ibis_step_cell.hpp"
 
 namespace {
+
+// Value every workspace byte starts from: IBIS_HOSTCHECK_WS_FILL=nan fills it with NaN, so a
+// test can show that the step never reads a workspace value it has not written.
+template <class T> struct scalar_of { using type = T; };
+template <class T> struct scalar_of<ibis::Dual<T>> { using type = T; };
+
+template <class T> void ws_fill(T* p, size_t n) {
+  const char* e = getenv("IBIS_HOSTCHECK_WS_FILL");
+  if (!e || strcmp(e, "nan") != 0) return;
+  using U = typename scalar_of<T>::type;              // both halves of a dual number
+  U* u = (U*)p;
+  for (size_t k = 0; k < n * (sizeof(T) / sizeof(U)); ++k) u[k] = U(NAN);
+}
 
 template <class S, class R, class D>
 void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
@@ -19,6 +35,8 @@ void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
   // the three phases over all cells, in the GPU's kernel order
   std::vector<D> wd((size_t)ibis::WD_N * (size_t)pitch);
   std::vector<R> wr((size_t)ibis::WR_N * (size_t)pitch);
+  ws_fill(wd.data(), wd.size());
+  ws_fill(wr.data(), wr.size());
   auto work = [&](int64_t i) { return ibis::Work<R, D>{wd.data(), wr.data(), pitch, i}; };
   for (int64_t i = 0; i < ncell; ++i) {
     if (copy_in) {
@@ -156,6 +174,10 @@ void tuned_host(S* a, const ibis::TunedArgs& g, const IbisParams& base,
                 float* flux, double* loss) {
   std::vector<unsigned char> ws((size_t)ibis::work_bytes(g.pitch, (int)sizeof(R),
                                                          (int)sizeof(D)));
+  // the D planes then the R planes, as tuned_work lays them out
+  ws_fill((D*)ws.data(), (size_t)ibis::WD_N * (size_t)g.pitch);
+  ws_fill((R*)((D*)ws.data() + (size_t)ibis::WD_N * (size_t)g.pitch),
+          (size_t)ibis::WR_N * (size_t)g.pitch);
   for (int64_t t = 0; t < g.nthread; ++t)
     ibis::tuned_surface<S, R, D, Q>(a, ws.data(), g, ibis::tuned_thread_of(g, t), base, K);
   for (int64_t t = 0; t < g.nthread; ++t)

@@ -300,17 +300,17 @@ IBIS_HD IBIS_INLINE Fwet<R> fwetcal(R tu, R ts, R tl, const R w[6], const PT& P)
   return o;
 }
 
-// mix (interception.py:101-119)
+// mix (interception.py:101-119).  Written as selects, with the quotient formed over a
+// divisor of 1 where the flows vanish: the same values as the branch form, and the form the
+// device compiler turns into correct code in every instantiation (the branch form, in the
+// FP64 dual-number build, left the no-flow temperature at a stale value instead of zero).
 template <class R>
 IBIS_HD IBIS_INLINE void mix3(R x1, R t1, R x2, R t2, R x3, R t3, R eps, R& xm, R& tm) {
   const R s = x1 + x2 + x3;
-  if (r_abs(s) >= eps) {
-    tm = (t1 * x1 + t2 * x2 + t3 * x3) / s;
-    xm = s;
-  } else {
-    tm = R(0);
-    xm = R(0);
-  }
+  const bool flows = r_abs(s) >= eps;
+  const R t = (t1 * x1 + t2 * x2 + t3 * x3) / (flows ? s : R(1));
+  tm = flows ? t : R(0);
+  xm = flows ? s : R(0);
 }
 
 // twet3 (interception.py:126-156): the latch freezes twk at the first converged step, so

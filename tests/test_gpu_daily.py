@@ -51,9 +51,13 @@ def test_day_end_kernel_matches_the_host_build_and_the_oracle(prec, nsoilcall):
     f, day, td, sums, comp = day_case(4000, seed=12)
     st, da, acc, vx = gpu_day(1, f, day, td, sums, comp, prec, nsoilcall)
     hst, hda, hacc, hvx = host_day(1, f, day, td, sums, comp, prec, nsoilcall)
-    tol = 1e-12 if prec == "f64" else 1e-5       # device exp vs the host's libm
+    # device exp vs the host's libm.  FP64: relative to each value.  FP32: relative to the
+    # field's magnitude, at the 1e-4 bar, since the day's carbon balance (tco2mic, and the
+    # annual respiration it feeds) cancels pools of tens of kg C down to a flux of 1e-4
+    tol = 1e-12 if prec == "f64" else 1e-4
     for k, _ in DY.DAY_FIELDS:
-        scale = np.maximum(np.abs(hda[k]), 1e-6 * np.abs(hda[k]).max() + 1e-300)
+        floor = 1e-6 if prec == "f64" else 1.0
+        scale = np.maximum(np.abs(hda[k]), floor * np.abs(hda[k]).max() + 1e-300)
         err = (np.abs(da[k] - hda[k]) / scale).max()
         assert err <= tol, (k, err)
     t, ht = vx[L_TCO2MIC], hvx[L_TCO2MIC]
