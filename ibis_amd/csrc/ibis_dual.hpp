@@ -124,14 +124,17 @@ template <class Q> struct TunedParams : IbisParams {
 };
 
 // Build the tuned set of one pair: base values, the pair's tunable values tune[k * tpitch]
-// (k = IbisTune), and tangent 1 on component `dir` (dir < 0: no seed).
+// (k = IbisTune), and on component `dir` the tangent theta_dir (dir < 0: no seed).  Seeding
+// with the parameter's own value makes the tangent theta dX/dtheta, of the same scale as the
+// values whatever the parameter's units (Vmax is 1e-5, heat capacities 1e3), which keeps
+// FP32 tangents far from overflow and underflow; dL/dtheta is the loss tangent / theta.
 template <class Q>
 IBIS_HD IBIS_INLINE void tune_params(TunedParams<Q>& t, const IbisParams& base,
                                      const double* tune, int64_t tpitch, int dir) {
   static_cast<IbisParams&>(t) = base;
   auto q = [&](int k) -> Q {
     const double v = tune[(int64_t)k * tpitch];
-    if constexpr (is_dual<Q>::value) return Q(v, k == dir ? 1.0 : 0.0);
+    if constexpr (is_dual<Q>::value) return Q(v, k == dir ? v : 0.0);
     else return Q(v);
   };
   t.alpha3 = q(TUNE_ALPHA3); t.alpha4 = q(TUNE_ALPHA4);

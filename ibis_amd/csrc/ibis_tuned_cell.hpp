@@ -70,7 +70,7 @@ IBIS_HD IBIS_INLINE void tuned_canopy(S* a, void* ws, const TunedArgs& g, const 
     const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
     const int k = IBIS_TUNE_VMAX + g.row_pft[row];
     const double v = g.tune[(int64_t)k * g.npair + th.pair];
-    if constexpr (is_dual<Q>::value) vrow[j] = R(Q(v, k == th.dir ? 1.0 : 0.0));
+    if constexpr (is_dual<Q>::value) vrow[j] = R(Q(v, k == th.dir ? v : 0.0));
     else vrow[j] = R(v);
   }
   TunedParams<Q> P;

@@ -22,7 +22,8 @@
  * sensible heat H = fsena (W m-2, the total to the air after cascad2).  With `flux` given they
  * are stored (float, IBIS_NFLUX planes of `pitch`); with `target` given (float, IBIS_NFLUX
  * planes of `nsite`), sum_f ((flux_f - target_f) / scale_f)^2 is added to loss[t] (double)
- * and, in tangent mode, its derivative to loss[pitch + t].
+ * and, in tangent mode, theta_k dL/dtheta_k (the derivative along the seed theta_k e_k, k =
+ * dir0 + t / npair) to loss[pitch + t].
  *
  * No counterpart in the reference's step (its gradients come from jax.grad over the scan).
  * Stream-ordered; allocates nothing.

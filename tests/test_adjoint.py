@@ -120,7 +120,7 @@ def test_tangent_matches_central_differences_f64(case):
     n = tune.shape[1]
     _, lt, _ = hr.run(tune, hours, prec="f64", tangent=True, ndir=AD.NTUNE, dir0=0,
                       target=target)
-    g_ad = lt[1].reshape(AD.NTUNE, n)
+    g_ad = lt[1].reshape(AD.NTUNE, n) / tune
     step = 1e-4
     var = np.tile(tune, (1, 2 * AD.NTUNE))
     h = step * np.abs(tune)

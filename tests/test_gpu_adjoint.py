@@ -63,7 +63,7 @@ def test_tangent_matches_host_build_f64(gpu_case):
     hr, run, tune, target = gpu_case
     _, lt, _ = hr.run(tune, HOURS, prec="f64", tangent=True, ndir=AD.NTUNE, dir0=0,
                       target=target)
-    gh = lt[1].reshape(AD.NTUNE, NS)
+    gh = lt[1].reshape(AD.NTUNE, NS) / tune
     _, gg = run.tangent(_dev(tune), _dev(target), prec="f64")
     e = AD.gradient_errors(gg.cpu().numpy(), gh, tune)
     print("\nf64 tangent GPU vs host:", e)
