@@ -63,7 +63,7 @@ class IbisTuned(C.Structure):
     _fields_ = [("nsite", C.c_int64), ("npair", C.c_int64), ("ndir", C.c_int64),
                 ("dir0", C.c_int32), ("jday0", C.c_int32), ("seed", C.c_uint64),
                 ("hour", C.c_int64), ("dtime", C.c_double), ("nytimes", C.c_double),
-                ("flux_scale", C.c_double * NFLUX)]
+                ("flux_scale", C.c_double * NFLUX), ("site0", C.c_int64)]
 
 
 def _lib_tuned():
@@ -122,9 +122,10 @@ class Sites:
     nsite: int
     seed: int = 3
     jday0: int = 172
+    site0: int = 0          # the sites are cells site0 .. site0 + nsite - 1 of the seeded domain
 
     def __post_init__(self):
-        f, slot, lat = synth.make_cells(self.nsite, self.seed)
+        f, slot, lat = synth.make_cells(self.nsite, self.seed, cell0=self.site0)
         self.state64 = abi.pack(f, self.nsite, dtype=np.float64)      # (NPLANE, nsite)
         self.slot = np.ascontiguousarray(np.asarray(slot, dtype=np.int8).T)  # (NPFT, nsite)
         self.lat = np.asarray(lat, dtype=np.float64)
@@ -172,6 +173,7 @@ class TunedRunner:
         run = IbisTuned()
         run.nsite, run.npair, run.ndir, run.dir0 = self.sites.nsite, npair, ndir, dir0
         run.jday0, run.seed, run.hour = self.sites.jday0, self.sites.seed, hour
+        run.site0 = self.sites.site0
         run.dtime = self.dtime
         run.nytimes = float(hour if nytimes is None else nytimes)
         run.flux_scale[:] = list(FLUX_SCALE)

@@ -15,6 +15,12 @@
 #include "ibis_adjoint.h"
 #include "ibis_step.h"
 
+// Debugging hook applied to the result of every dual operation: empty in every product build;
+// a host debugging build may define it to report the first finite value whose tangent is not.
+#ifndef IBIS_DUAL_CHECK
+#define IBIS_DUAL_CHECK(r) (r)
+#endif
+
 namespace ibis {
 
 template <class T> struct Dual {
@@ -32,15 +38,15 @@ template <class T> struct Dual {
   template <class U, typename std::enable_if<std::is_arithmetic<U>::value, int>::type = 0>
   IBIS_HD IBIS_INLINE explicit operator U() const { return U(v); }
 
-  IBIS_HD IBIS_INLINE friend Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-  IBIS_HD IBIS_INLINE friend Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+  IBIS_HD IBIS_INLINE friend Dual operator+(Dual a, Dual b) { return IBIS_DUAL_CHECK((Dual{a.v + b.v, a.d + b.d})); }
+  IBIS_HD IBIS_INLINE friend Dual operator-(Dual a, Dual b) { return IBIS_DUAL_CHECK((Dual{a.v - b.v, a.d - b.d})); }
   IBIS_HD IBIS_INLINE friend Dual operator-(Dual a) { return {-a.v, -a.d}; }
   IBIS_HD IBIS_INLINE friend Dual operator*(Dual a, Dual b) {
-    return {a.v * b.v, a.d * b.v + a.v * b.d};
+    return IBIS_DUAL_CHECK((Dual{a.v * b.v, a.d * b.v + a.v * b.d}));
   }
   IBIS_HD IBIS_INLINE friend Dual operator/(Dual a, Dual b) {
     const T q = a.v / b.v;
-    return {q, (a.d - q * b.d) / b.v};
+    return IBIS_DUAL_CHECK((Dual{q, (a.d - q * b.d) / b.v}));
   }
   IBIS_HD IBIS_INLINE Dual& operator+=(Dual b) { return *this = *this + b; }
   IBIS_HD IBIS_INLINE Dual& operator-=(Dual b) { return *this = *this - b; }
@@ -54,29 +60,27 @@ template <class T> struct Dual {
   IBIS_HD IBIS_INLINE friend bool operator!=(Dual a, Dual b) { return a.v != b.v; }
 };
 
-template <class T> struct is_dual : std::false_type {};
-template <class T> struct is_dual<Dual<T>> : std::true_type {};
 
 // Elementary functions: value by the scalar routine, tangent by the chain rule.  A zero input
 // tangent gives a zero output tangent even where the derivative is infinite (sqrt at 0, pow
 // at a zero base), so an unseeded quantity never turns into NaN.
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_exp(Dual<T> a) {
   const T e = r_exp(a.v);
-  return {e, e * a.d};
+  return IBIS_DUAL_CHECK((Dual<T>{e, e * a.d}));
 }
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_log(Dual<T> a) {
-  return {r_log(a.v), a.d / a.v};
+  return IBIS_DUAL_CHECK((Dual<T>{r_log(a.v), a.d / a.v}));
 }
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_sqrt(Dual<T> a) {
   const T s = r_sqrt(a.v);
-  return {s, a.d == T(0) ? T(0) : a.d / (T(2) * s)};
+  return IBIS_DUAL_CHECK((Dual<T>{s, a.d == T(0) ? T(0) : a.d / (T(2) * s)}));
 }
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_pow(Dual<T> a, Dual<T> b) {
   const T p = r_pow(a.v, b.v);
   T d = T(0);
   if (a.d != T(0)) d = d + b.v * r_pow(a.v, b.v - T(1)) * a.d;
   if (b.d != T(0)) d = d + p * r_log(a.v) * b.d;
-  return {p, d};
+  return IBIS_DUAL_CHECK((Dual<T>{p, d}));
 }
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_abs(Dual<T> a) {
   return {r_abs(a.v), a.v < T(0) ? -a.d : a.d};

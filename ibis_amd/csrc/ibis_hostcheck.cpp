@@ -208,6 +208,7 @@ extern "C" int ibis_hostcheck_tuned_hour(void* state, int64_t pitch, const int8_
   g.nsite = run->nsite;
   g.npair = run->npair;
   g.nthread = run->ndir * run->npair;
+  g.site0 = run->site0;
   g.dir0 = tangent ? run->dir0 : -1;
   g.jday0 = run->jday0;
   g.seed = run->seed;

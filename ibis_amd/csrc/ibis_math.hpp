@@ -18,6 +18,12 @@
 
 namespace ibis {
 
+// Dual numbers (value, tangent) of ibis_dual.hpp, and the trait that tells them apart, so that
+// precision-generic code can give a derivative a better-conditioned form than the chain rule.
+template <class T> struct Dual;
+template <class T> struct is_dual { static constexpr bool value = false; };
+template <class T> struct is_dual<Dual<T>> { static constexpr bool value = true; };
+
 IBIS_HD IBIS_INLINE float r_exp(float x) { return expf(x); }
 IBIS_HD IBIS_INLINE double r_exp(double x) { return exp(x); }
 IBIS_HD IBIS_INLINE float r_log(float x) { return logf(x); }

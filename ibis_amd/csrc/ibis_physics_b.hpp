@@ -19,12 +19,29 @@ namespace ibis {
 // ---------------------------------------------------------------------------------------
 
 // colimit (stomata.py:198-221): smaller root of theta x^2 - (j1 + j2) x + j1 j2
+//
+// In dual numbers, where the chain rule through sqrt(b^2 - 4 theta c) gives a non-finite
+// tangent -- the discriminant rounds to zero in FP32 when b^2 and 4 theta c agree to its
+// precision, though the roots are real and distinct -- the tangent is that of the exact root
+// by implicit differentiation, dx = -[x (dj1 + dj2) - j2 dj1 - j1 dj2 - x^2 dtheta] / sqrt(D),
+// with D = (j1 - j2)^2 + 4 (1 - theta) j1 j2, the same discriminant written without
+// cancellation.  The value, and the tangent everywhere else, are the chain rule's.
 template <class R> IBIS_HD IBIS_INLINE R colimit(R j1, R j2, R theta) {
   const R b = j1 + j2;
   const R c = j1 * j2;
   const R disc = mx(b * b - R(4) * theta * c, R(0));
   const R q = R(0.5) * (b + r_sqrt(disc)) + R(1.0e-15);
-  return mn(q / theta, c / q);
+  R x = mn(q / theta, c / q);
+  if constexpr (is_dual<R>::value) {
+    using T = decltype(x.v);
+    const T a1 = j1.v, a2 = j2.v, th = theta.v, xv = x.v;
+    const T dd = (a1 - a2) * (a1 - a2) + T(4) * (T(1) - th) * a1 * a2;
+    if (!(x.d - x.d == T(0)) && dd > T(0)) {        // tangent not finite, roots distinct
+      const T num = xv * (j1.d + j2.d) - a2 * j1.d - a1 * j2.d - xv * xv * theta.d;
+      x.d = -num / r_sqrt(dd);
+    }
+  }
+  return x;
 }
 
 template <class R> struct Leaf { R ag, an; };

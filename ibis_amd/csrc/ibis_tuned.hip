@@ -52,9 +52,10 @@ int ibis_tuned_hour(const IbisArena* state, void* workspace, int64_t workspace_b
     return fail(IBIS_E_FLAGS, "ibis_tuned_hour: unknown flag bits, or precision other than "
                               "mixed or FP64");
   const IbisTuned& r = *run;
-  if (r.nsite <= 0 || r.npair <= 0 || r.ndir <= 0 || r.npair % r.nsite != 0 || r.hour < 0)
+  if (r.nsite <= 0 || r.npair <= 0 || r.ndir <= 0 || r.npair % r.nsite != 0 || r.hour < 0 ||
+      r.site0 < 0)
     return fail(IBIS_E_ARG, "ibis_tuned_hour: nsite, npair, ndir must be positive, npair a "
-                            "multiple of nsite, hour >= 0");
+                            "multiple of nsite, hour and site0 >= 0");
   if (tangent ? (r.dir0 < 0 || r.dir0 + r.ndir > IBIS_NTUNE) : (r.dir0 != -1 || r.ndir != 1))
     return fail(IBIS_E_ARG, "ibis_tuned_hour: tangent mode needs 0 <= dir0 and dir0 + ndir <= "
                             "IBIS_NTUNE; value mode needs dir0 = -1 and ndir = 1");
@@ -86,6 +87,7 @@ int ibis_tuned_hour(const IbisArena* state, void* workspace, int64_t workspace_b
   g.nsite = r.nsite;
   g.npair = r.npair;
   g.nthread = nthread;
+  g.site0 = r.site0;
   g.dir0 = tangent ? r.dir0 : -1;
   g.jday0 = r.jday0;
   g.seed = r.seed;

@@ -13,7 +13,7 @@ namespace ibis {
 
 // Per-launch arguments.
 struct TunedArgs {
-  int64_t pitch, nsite, npair, nthread;
+  int64_t pitch, nsite, npair, nthread, site0;
   int dir0, jday0;
   uint64_t seed;
   int64_t hour;
@@ -51,7 +51,7 @@ IBIS_HD IBIS_INLINE void tuned_surface(S* a, void* ws, const TunedArgs& g, const
                                        const IbisParams& base, const StepConst& K) {
   Forcing f;
   synthetic_forcing(f, g.lat[th.site], (double)a[(int64_t)IBIS_P_ATM_PSURF * g.pitch + th.t],
-                    g.seed, (uint64_t)th.site, g.jday0, g.hour);
+                    g.seed, (uint64_t)(g.site0 + th.site), g.jday0, g.hour);
   put_forcing(a, g.pitch, th.t, f);
   TunedParams<Q> P;
   tune_params(P, base, g.tune + th.pair, g.npair, th.dir);

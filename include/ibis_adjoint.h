@@ -6,7 +6,7 @@
  * Threads.  A launch covers ndir x npair threads; thread t = k * npair + p steps pair p
  * (0 <= p < npair) and, in tangent mode, carries the derivative along tunable component
  * dir0 + k.  Pair p runs site p % nsite: the site's latitude, slot types, forcing (the
- * benchmark generator of ibis_forcing_synthetic keyed by seed, site index and hour) and
+ * benchmark generator of ibis_forcing_synthetic keyed by seed, site0 + site index and hour) and
  * target fluxes; its parameter vector is column p of `tune`.  Running variants of a site's
  * vector as extra pairs (npair = nsite x variants) evaluates finite differences in one launch.
  *
@@ -58,6 +58,8 @@ typedef struct IbisTuned {
   int64_t hour;                /* hour index of this step (forcing) */
   double dtime, nytimes;       /* step length; VegetationState.nytimes before the step */
   double flux_scale[IBIS_NFLUX];
+  int64_t site0;               /* forcing key of site 0: site i draws the forcing of cell
+                                  site0 + i (a window of a larger seeded domain) */
 } IbisTuned;
 
 /* Workspace bytes for arenas of `pitch` threads under `flags` (precision | TANGENT). */

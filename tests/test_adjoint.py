@@ -25,8 +25,8 @@ def _lib():
 class HostRun:
     """Hourly value / tangent runs of the host build over numpy arrays."""
 
-    def __init__(self, nsite, seed=3):
-        self.sites = AD.Sites(nsite, seed)
+    def __init__(self, nsite, seed=3, site0=0):
+        self.sites = AD.Sites(nsite, seed, site0=site0)
         self.base, _ = AD.base_params()
         self.rows = AD.row_pft()
 
@@ -46,6 +46,7 @@ class HostRun:
         run = AD.IbisTuned()
         run.nsite, run.npair, run.ndir, run.dir0 = n, npair, ndir, dir0
         run.jday0, run.seed, run.dtime = self.sites.jday0, self.sites.seed, 3600.0
+        run.site0 = self.sites.site0
         run.flux_scale[:] = list(AD.FLUX_SCALE)
         flags = (FLAGS_F64 if prec == "f64" else FLAGS_MIXED) | (AD.FLAG_TANGENT if tangent else 0)
         ptr = lambda x: x.ctypes.data_as(C.c_void_p) if x is not None else None
