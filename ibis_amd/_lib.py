@@ -7,13 +7,14 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import IbisArena, IbisParams
+from .abi import IbisArena, IbisParams, IbisStomataExt
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libibis_step.so")
 
 #: every entry point declared in include/*.h
-EXPORTS = ("ibis_step", "ibis_step_workspace_bytes", "ibis_last_error", "ibis_version",
+EXPORTS = ("ibis_step", "ibis_step_ext", "ibis_step_workspace_bytes", "ibis_last_error",
+           "ibis_version",
            "ibis_forcing_synthetic", "ibis_forcing_diurnal", "ibis_accumulate",
            "ibis_serve_tick", "ibis_tuned_workspace_bytes", "ibis_tuned_hour",
            "ibis_day_begin", "ibis_day_end")
@@ -38,6 +39,11 @@ def load():
                               C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(IbisParams),
                               C.c_double, C.c_double, C.c_int64, C.c_int, C.c_void_p]
     lib.ibis_step.restype = C.c_int
+    lib.ibis_step_ext.argtypes = [C.POINTER(IbisArena), C.POINTER(IbisArena), C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.c_int64, C.POINTER(IbisParams),
+                                  C.POINTER(IbisStomataExt), C.c_double, C.c_double, C.c_int64,
+                                  C.c_int, C.c_void_p]
+    lib.ibis_step_ext.restype = C.c_int
     lib.ibis_step_workspace_bytes.argtypes = [C.c_int64, C.c_int]
     lib.ibis_step_workspace_bytes.restype = C.c_int64
     lib.ibis_last_error.restype = C.c_char_p

@@ -21,6 +21,20 @@ FLAG_COPY_IN = 0x20
 NTYPE = 40
 TYPE_ROWS = NTYPE + L.NPFT
 
+# optional StomataParams switches (include/ibis_step.h, python-port/ibis/physics/stomata.py:93-161)
+SW_VMAX_MIX, SW_COEFM_MIX, SW_SUNSHADE_U, SW_SUNSHADE_L = 0x001, 0x002, 0x004, 0x008
+SW_CAP_SCALE_U, SW_CAP_SCALE_L, SW_SCALE_LEAF_ONLY = 0x010, 0x020, 0x040
+SW_ACCLIM, SW_ACCLIM_PER_PFT, SW_ACCLIM_LIGHT, SW_FROST = 0x080, 0x100, 0x200, 0x400
+SW_ALL = 0x7ff
+#: StomataParams switch name -> bit (acclim_on maps to SW_ACCLIM)
+SWITCHES = {"vmax_mix": SW_VMAX_MIX, "coefm_mix": SW_COEFM_MIX, "sunshade_u": SW_SUNSHADE_U,
+            "sunshade_l": SW_SUNSHADE_L, "cap_scale_u": SW_CAP_SCALE_U,
+            "cap_scale_l": SW_CAP_SCALE_L, "scale_leaf_only": SW_SCALE_LEAF_ONLY,
+            "acclim_on": SW_ACCLIM, "acclim_per_pft": SW_ACCLIM_PER_PFT,
+            "acclim_light": SW_ACCLIM_LIGHT, "frost_on": SW_FROST}
+#: per-slot tables of ibis_step_ext, in order
+NTABLE = 4
+
 _D = C.c_double
 
 
@@ -55,6 +69,32 @@ class IbisParams(C.Structure):
     )
 
 
+class IbisStomataExt(C.Structure):
+    _fields_ = [("switches", C.c_int32), ("acclim_mask", C.c_int32), ("cap_kn", _D),
+                ("acclim_s0", _D), ("acclim_smax", _D), ("frost_t0", _D), ("frost_t1", _D),
+                ("type_tables", C.c_void_p), ("acclim_x", C.c_void_p), ("td", C.c_void_p),
+                ("stressfac", C.c_void_p)]
+
+
+def switch_bits(sp) -> int:
+    """IBIS_SW_* bits of a StomataParams-like object (absent attributes are off)."""
+    return sum(bit for name, bit in SWITCHES.items() if bool(getattr(sp, name, False)))
+
+
+def stomata_ext(sp, type_tables=0, acclim_x=0, td=0, stressfac=0) -> IbisStomataExt:
+    """IbisStomataExt from a StomataParams-like object and device pointers."""
+    x = IbisStomataExt()
+    x.switches = switch_bits(sp)
+    x.acclim_mask = sum(1 << int(j) for j in getattr(sp, "acclim_pfts", ()))
+    x.cap_kn = float(getattr(sp, "cap_kn", 0.15))
+    x.acclim_s0 = float(getattr(sp, "acclim_s0", -3.063))
+    x.acclim_smax = float(getattr(sp, "acclim_smax", 17.72))
+    x.frost_t0 = float(getattr(sp, "frost_t0", -5.0))
+    x.frost_t1 = float(getattr(sp, "frost_t1", 5.0))
+    x.type_tables, x.acclim_x, x.td, x.stressfac = type_tables, acclim_x, td, stressfac
+    return x
+
+
 def params_from(consts, sim, snow, hsoi, froot, co2conc, flux_max=2000.0, t_max_dev=40.0):
     """Fill IbisParams from reference-named parameter objects.
 
@@ -74,9 +114,9 @@ def params_from(consts, sim, snow, hsoi, froot, co2conc, flux_max=2000.0, t_max_
         v = np.asarray(getattr(sp, n), dtype=np.float64)
         getattr(p, n)[:] = list(v)
     sf = np.asarray(sp.stressfac, dtype=np.float64)
-    if sf.ndim != 0:
-        raise ValueError("per-cell stressfac is not supported by this build; pass a scalar")
-    p.stressfac = float(sf)
+    # a per-cell stressfac travels as its own plane (IbisStomataExt.stressfac); the scalar
+    # slot is then never read and holds NaN
+    p.stressfac = float(sf) if sf.ndim == 0 else float("nan")
     cp = sim.canopy
     for n in ("cleaf", "cstem", "cgrass", "alaiml", "alaimu", "chu", "chs", "chl",
               "soil_beta_power"):
@@ -117,6 +157,12 @@ def params_from(consts, sim, snow, hsoi, froot, co2conc, flux_max=2000.0, t_max_
         p.froot[k][0], p.froot[k][1] = float(fr[k, 0]), float(fr[k, 1])
     p.co2conc = float(co2conc)
     return p
+
+
+def params_bytes(p: "IbisParams"):
+    """The IbisParams block as a CPU uint8 tensor (the ``params`` argument of torch.ops.ibis.step)."""
+    import torch
+    return torch.frombuffer(bytearray(bytes(p)), dtype=torch.uint8)
 
 
 def pack(fields: dict, n: int, dtype=np.float32, pitch: int | None = None, fill=0.0):

@@ -68,6 +68,9 @@ template <class T> IBIS_HD IBIS_INLINE Dual<T> r_exp(Dual<T> a) {
   const T e = r_exp(a.v);
   return IBIS_DUAL_CHECK((Dual<T>{e, e * a.d}));
 }
+template <class T> IBIS_HD IBIS_INLINE Dual<T> r_expm1(Dual<T> a) {
+  return IBIS_DUAL_CHECK((Dual<T>{r_expm1(a.v), r_exp(a.v) * a.d}));
+}
 template <class T> IBIS_HD IBIS_INLINE Dual<T> r_log(Dual<T> a) {
   return IBIS_DUAL_CHECK((Dual<T>{r_log(a.v), a.d / a.v}));
 }

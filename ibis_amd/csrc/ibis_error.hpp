@@ -16,4 +16,9 @@ inline int fail(int code, const char* msg) {
   return code;
 }
 
+template <class A, class... B> inline int fail(int code, const char* fmt, A a, B... b) {
+  snprintf(err_buf(), 512, fmt, a, b...);
+  return code;
+}
+
 }  // namespace ibis

@@ -28,10 +28,11 @@ template <class T> void ws_fill(T* p, size_t n) {
   for (size_t k = 0; k < n * (sizeof(T) / sizeof(U)); ++k) u[k] = U(NAN);
 }
 
-template <class S, class R, class D>
+// EXT: the optional StomataParams switches of E, with `table` holding IBIS_NTABLE tables.
+template <class S, class R, class D, bool EXT = false>
 void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
          int64_t slot_pitch, const double* table, const IbisParams& P, const ibis::StepConst& K,
-         int copy_in) {
+         int copy_in, const ibis::StepExt* E = nullptr) {
   // the three phases over all cells, in the GPU's kernel order
   std::vector<D> wd((size_t)ibis::WD_N * (size_t)pitch);
   std::vector<R> wr((size_t)ibis::WR_N * (size_t)pitch);
@@ -48,17 +49,19 @@ void run(const S* in, S* out, int64_t pitch, int64_t ncell, const int8_t* slot,
 #undef X
     }
     const ibis::Cell<S> c{in, out, pitch, i};
-    ibis::surface_cell<S, R, D>(c, work(i), P, K);
+    ibis::surface_cell_x<S, R, D, IbisParams, EXT>(c, work(i), P, K, E);
   }
   for (int64_t i = 0; i < ncell; ++i) {
-    R vrow[IBIS_NPFT];
+    R vrow[IBIS_NPFT], xrow[3][IBIS_NPFT];
     for (int j = 0; j < IBIS_NPFT; ++j) {
       const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
       const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
       vrow[j] = R(table[row]);
+      if (EXT)
+        for (int q = 0; q < 3; ++q) xrow[q][j] = R(table[(1 + q) * IBIS_TYPE_ROWS + row]);
     }
     const ibis::Cell<S> c{in, out, pitch, i};
-    ibis::canopy_cell<S, R, D>(c, work(i), vrow, P, K);
+    ibis::canopy_cell_x<S, R, D, IbisParams, EXT>(c, work(i), vrow, P, K, E, xrow);
   }
   for (int64_t i = 0; i < ncell; ++i) {
     const ibis::Cell<S> c{in, out, pitch, i};
@@ -90,6 +93,39 @@ extern "C" int ibis_hostcheck_step(const void* in, void* out, int64_t pitch, int
     case IBIS_PREC_F64:
       run<double, double, double>((const double*)in, (double*)out, pitch, ncell, slot,
                                   slot_pitch, table, *P, K, copy_in);
+      return 0;
+    default:
+      return -2;
+  }
+}
+
+// ibis_step_ext on the host: `ext` as for the device entry point, its pointers host memory
+// (the entry point's own switch normalisation applied here too)
+extern "C" int ibis_hostcheck_step_ext(const void* in, void* out, int64_t pitch, int64_t ncell,
+                                       const int8_t* slot, int64_t slot_pitch,
+                                       const IbisParams* P, const IbisStomataExt* ext,
+                                       double dtime, double nytimes, int flags) {
+  ibis::StepConst K;
+  K.dtime = dtime;
+  K.tfac = 1.0 / pow(0.999, P->cappa);
+  K.zweight = exp(-1.0 / (10.0 * 86400.0 / dtime));
+  K.nytimes_new = nytimes + 1.0;
+  K.bedrock = (flags & IBIS_FLAG_BEDROCK) ? 1 : 0;
+  const int copy_in = (flags & IBIS_FLAG_COPY_IN) ? 1 : 0;
+  const ibis::StepExt E = ibis::step_ext_from(*ext);
+  const double* tab = ext->type_tables;
+  switch (flags & IBIS_PREC_MASK) {
+    case IBIS_PREC_MIXED:
+      run<float, float, double, true>((const float*)in, (float*)out, pitch, ncell, slot,
+                                      slot_pitch, tab, *P, K, copy_in, &E);
+      return 0;
+    case IBIS_PREC_F32:
+      run<float, float, float, true>((const float*)in, (float*)out, pitch, ncell, slot,
+                                     slot_pitch, tab, *P, K, copy_in, &E);
+      return 0;
+    case IBIS_PREC_F64:
+      run<double, double, double, true>((const double*)in, (double*)out, pitch, ncell, slot,
+                                        slot_pitch, tab, *P, K, copy_in, &E);
       return 0;
     default:
       return -2;

@@ -26,6 +26,8 @@ template <class T> struct is_dual<Dual<T>> { static constexpr bool value = true;
 
 IBIS_HD IBIS_INLINE float r_exp(float x) { return expf(x); }
 IBIS_HD IBIS_INLINE double r_exp(double x) { return exp(x); }
+IBIS_HD IBIS_INLINE float r_expm1(float x) { return expm1f(x); }
+IBIS_HD IBIS_INLINE double r_expm1(double x) { return expm1(x); }
 IBIS_HD IBIS_INLINE float r_log(float x) { return logf(x); }
 IBIS_HD IBIS_INLINE double r_log(double x) { return log(x); }
 IBIS_HD IBIS_INLINE float r_sqrt(float x) { return sqrtf(x); }

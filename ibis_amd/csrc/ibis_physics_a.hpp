@@ -146,6 +146,7 @@ template <class R> struct RadOut {
   R asurd[2], asuri[2];
   R topparu, topparl, scalcoefu[4], scalcoefl[4];
   R termu6, termu7, terml6, terml7;
+  R dbeamu, dbeaml;   // unscattered beam PAR absorbed per unit plant area at each canopy top
 };
 
 template <class R> struct Fwet { R fwetu, fwets, fwetl, rliqu, rliqs, rliql; };
@@ -161,6 +162,7 @@ IBIS_HD void radiation(RadOut<R>& o, R coszen, const R solad[2], const R solai[2
   const bool sunlit = coszen > R(0);
   o.solu = o.sols = o.soll = o.solg = o.soli = R(0);
   o.topparu = o.topparl = R(0);
+  o.dbeamu = o.dbeaml = R(0);
   for (int j = 0; j < 4; ++j) { o.scalcoefu[j] = R(0); o.scalcoefl[j] = R(0); }
 #pragma unroll
   for (int ib = 0; ib < 2; ++ib) {
@@ -239,6 +241,9 @@ IBIS_HD void radiation(RadOut<R>& o, R coszen, const R solad[2], const R solai[2
       o.topparl = l3 * lai[0] / xail;
       o.termu6 = up.term[5]; o.termu7 = up.term[6];
       o.terml6 = lo.term[5]; o.terml7 = lo.term[6];
+      // (1 - omega) k I_beam at the top of each canopy (radiation.py:516, :527)
+      o.dbeamu = (R(1) - up.omega) * up.term[5] * sd;
+      o.dbeaml = (R(1) - lo.omega) * lo.term[5] * sol3d;
     }
   }
 }

@@ -70,6 +70,119 @@ IBIS_HD IBIS_INLINE Leaf<R> farq_c4(R par, R ci, R vmax, R rdark, const PT& P) {
   return o;
 }
 
+// Leaf rates with the optional factors: `jef` scales the light-limited rate (acclim_light; 1
+// otherwise); with `cap`, the light limit is scaled by `fl` and the Rubisco and sink limits and
+// dark respiration by `fc` before co-limitation (cap_scale; stomata.py:253-268, :299-310).
+template <class R, class PT>
+IBIS_HD IBIS_INLINE Leaf<R> farq_c3_x(R par, R ci, R vmax, R gstar, R kc, R ko, R o2, R rdark,
+                                      const PT& P, R jef, bool cap, R fl, R fc) {
+  R je = par * R(4.59e-06) * R(P.alpha3) * (ci - gstar) / (ci + R(2) * gstar) * jef;
+  R jc = vmax * (ci - gstar) / (ci + kc * (R(1) + o2 / ko));
+  R js = vmax / R(2.2);
+  if (cap) {
+    je = je * fl;
+    jc = jc * fc;
+    js = js * fc;
+    rdark = rdark * fc;
+  }
+  Leaf<R> o;
+  o.ag = colimit(colimit(je, jc, R(P.theta3)), js, R(P.beta3));
+  o.an = o.ag - rdark;
+  return o;
+}
+
+template <class R, class PT>
+IBIS_HD IBIS_INLINE Leaf<R> farq_c4_x(R par, R ci, R vmax, R rdark, const PT& P, R jef, bool cap,
+                                      R fl, R fc) {
+  R je = par * R(4.59e-06) * R(P.alpha4) * jef;
+  R jc = vmax;
+  R ji = R(18.0e3) * vmax * ci;
+  if (cap) {
+    je = je * fl;
+    jc = jc * fc;
+    ji = ji * fc;
+    rdark = rdark * fc;
+  }
+  Leaf<R> o;
+  o.ag = colimit(colimit(je, jc, R(P.theta4)), ji, R(P.beta4));
+  o.an = o.ag - rdark;
+  return o;
+}
+
+// Integral of e^-a x over [0, plai]: (1 - e^-a plai) / a, and plai (1 - a plai / 2) where
+// |a plai| < 1e-5 (stomata.py:387-392)
+template <class R> IBIS_HD IBIS_INLINE R e1(R a, R plai) {
+  const R x = a * plai;
+  const bool small = r_abs(x) < R(1.0e-5);
+  const R wide = -r_expm1(-x) / (small ? R(1) : a);
+  return small ? plai * (R(1) - R(0.5) * x) : wide;
+}
+
+// Canopy-mean relative capacity of an exponential capacity profile, (1 - e^-Kn L) / (Kn L)
+// (stomata.py:379-384)
+template <class R> IBIS_HD IBIS_INLINE R capacity_integral(R lai, R kn) {
+  const R x = kn * lai;
+  const R f = x > R(1.0e-4) ? -r_expm1(-x) / mx(x, R(1.0e-4)) : R(1) - R(0.5) * x;
+  return lai > R(0) ? f : R(0);
+}
+
+// Sunlit / shaded partition of one canopy (stomata.py:395-424): absorbed PAR and capacity (in
+// top-leaf units) of the sunlit and shaded leaves per unit ground, and the per-leaf-area
+// factor of the canopy-mean rate (stomata.py:427-443).
+template <class R> struct SunShade { R a_sun, a_sha, v_sun, v_sha, inv; };
+
+template <class R>
+IBIS_HD IBIS_INLINE SunShade<R> sunshade_canopy(const R sc[4], R dbeam, R k, R h, R lai, R sai,
+                                                R kn, R eps) {
+  const R plai = mx(lai + sai, eps);
+  const R fl = lai > R(0) ? lai / plai : R(0);
+  const R ek = e1(k, plai), e2k = e1(R(2) * k, plai);
+  const R tot = mx(sc[0] * ek + sc[1] * e1(h, plai) + sc[2] * e1(-h, plai), R(0));
+  R sun = sc[0] * e2k + sc[1] * e1(k + h, plai) + sc[2] * e1(k - h, plai) + dbeam * (ek - e2k);
+  sun = clip(sun, R(0), tot);
+  SunShade<R> o;
+  o.a_sun = sun * fl;
+  o.a_sha = (tot - sun) * fl;
+  o.v_sun = fl * e1(kn * fl + k, plai);
+  o.v_sha = mx(fl * e1(kn * fl, plai) - o.v_sun, R(0));
+  o.inv = lai > R(0) ? R(1) / mx(lai, R(1.0e-6)) : R(0);
+  return o;
+}
+
+// Displayed-LAI-weighted mean of v over the positions of `mask`, `fallback` where they hold no
+// leaves (stomata.py:587-595)
+template <class R>
+IBIS_HD IBIS_INLINE R mix_pft(const R frac[IBIS_NPFT], const R v[IBIS_NPFT], int mask,
+                              R fallback) {
+  R tot = R(0), num = R(0);
+  bool first = true;
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    if (mask & (1 << j)) {
+      tot = first ? frac[j] : tot + frac[j];
+      num = first ? frac[j] * v[j] : num + frac[j] * v[j];
+      first = false;
+    }
+  }
+  const R eps = R(1.0e-6);
+  return tot > eps ? num / mx(tot, eps) : fallback;
+}
+
+// plant-type positions of the mixed big leaves (stomata.py:569-571)
+constexpr int kBroadleaf = (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 6);
+constexpr int kConifer = (1 << 3) | (1 << 5) | (1 << 7);
+constexpr int kShrub = (1 << 8) | (1 << 9);
+
+// Per-cell, pass-invariant inputs of the five big leaves (ub uc ls l3 l4) under the optional
+// switches: nominal Vmax, Ball-Berry slope, light-rate factor, sunlit/shaded partitions of
+// the upper and lower canopy and their capacity integrals.
+template <class R> struct StomExt {
+  int sw;
+  R vnom[5], cm[5], je[5];
+  SunShade<R> ss[2];
+  R fcap[2];
+};
+
 // ball_berry_step (stomata.py:321-372); cs/gs/ci updated in place
 template <class R>
 IBIS_HD IBIS_INLINE void ball_berry(R an, R& cs, R& gs, R& ci, R co2, R gb, R rh, R coefm,
@@ -85,17 +198,18 @@ IBIS_HD IBIS_INLINE void ball_berry(R an, R& cs, R& gs, R& ci, R co2, R gb, R rh
   ci = i;
 }
 
-// canopy_scaling (stomata.py:442-508), F77 denominator
+// canopy_scaling (stomata.py:442-508): F77 divides the absorbed fraction by k L,
+// scale_leaf_only by k (L + S)
 template <class R>
 IBIS_HD IBIS_INLINE R canopy_scaling(R toppar, R t6, R t7, const R sc[4], R lai, R sai, R& a10s,
-                                     R& a10d, R zw, R eps) {
+                                     R& a10d, R zw, R eps, bool leaf_only = false) {
   R ext = (t6 * sc[0] + t7 * sc[1] - t7 * sc[2]) / mx(sc[3], eps);
   ext = clip(ext, R(0.1), R(10));
   const R pxai = ext * (lai + sai);
   const R plai = ext * lai;
   const bool has = plai > R(0);
   const bool day = toppar > R(10);
-  const R sday = (R(1) - r_exp(-pxai)) / mx(plai, eps);
+  const R sday = (R(1) - r_exp(-pxai)) / mx(leaf_only ? pxai : plai, eps);
   const R snight = a10s / mx(a10d, eps);
   if (has && day) {
     a10s = zw * a10s + (R(1) - zw) * sday * toppar;
@@ -136,38 +250,84 @@ template <class R> struct Stom {
   R a10[4];                // scalparamu, daylightu, scalparaml, daylightl
 };
 
-// stomata (stomata.py:732-1080); vmax6 = (pft 0, 2, 4, 3, 5, 8, 10, 11) nominal values
-template <class R, class PT>
-IBIS_HD void stomata(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R sl, R psurf,
-                     const RadOut<R>& rad, R fwetu, R fwetl, const R lai[2], const R sai[2],
-                     R vub, R vuc, R vls, R vl3, R vl4, R stresstu, R stresstl, R co2, R zw,
-                     const PT& P) {
+// stomata (stomata.py:732-1080).  vnom: nominal Vmax of ub uc ls l3 l4.  EXT adds the optional
+// switches of X (sunshade, cap_scale, scale_leaf_only; slopes and light factors per big leaf);
+// without it the slopes are the group values and the canopy rates the F77 scaled leaf rates.
+template <class R, class PT, bool EXT>
+IBIS_HD void stomata_impl(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R sl,
+                          R psurf, const RadOut<R>& rad, R fwetu, R fwetl, const R lai[2],
+                          const R sai[2], const R vnom[5], R stresstu, R stresstl, R co2, R zw,
+                          const PT& P, const StomExt<R>* X) {
   const R eps = R(P.epsilon), cimax = R(P.cimax), o2 = R(P.o2conc);
+  bool leaf_only = false, sun_u = false, sun_l = false, cap_u = false, cap_l = false;
+  if constexpr (EXT) {
+    leaf_only = (X->sw & IBIS_SW_SCALE_LEAF_ONLY) != 0;
+    sun_u = (X->sw & IBIS_SW_SUNSHADE_U) != 0;
+    sun_l = (X->sw & IBIS_SW_SUNSHADE_L) != 0;
+    cap_u = !sun_u && (X->sw & IBIS_SW_CAP_SCALE_U) != 0;
+    cap_l = !sun_l && (X->sw & IBIS_SW_CAP_SCALE_L) != 0;
+  }
+  // Ball-Berry slope of big leaf k (group g): the group value unless the switches set it
+  auto slope = [&](int k, int g) -> R {
+    if constexpr (EXT) return X->cm[k];
+    else return R(P.coefm[g]);
+  };
   // upper canopy
   TempDep<R> td = temp_dep(tu, P);
   R gmin = R(1.05) * td.gstar;
   R gb = clip(su * R(25.5), R(0.1), R(10));
   const R rh12 = mx(R(0.30), q12 / mx(qsat(esat(t12), psurf), eps));
-  const R vnom_u[2] = {vub, vuc};
-  Leaf<R> lf[5];
+  Leaf<R> lf[5], cr[5];          // leaf rates; canopy rates of the sunlit/shaded canopies
+  R ci0[5], vmx[5], rdk[5];      // what this pass's rates used (cap_scale)
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
     s.ci[g] = clip(s.ci[g], gmin, cimax);
-    const R vmax = vnom_u[g] * td.tv3 * stresstu;
-    const R rd = R(P.gamma[g]) * vnom_u[g] * td.tv3;
-    lf[g] = farq_c3(rad.topparu, s.ci[g], vmax, td.gstar, td.kc, td.ko, o2, rd, P);
-    ball_berry(lf[g].an, s.cs[g], s.gs[g], s.ci[g], co2, gb, rh12, R(P.coefm[g]), R(P.coefb[g]),
+    const R vmax = vnom[g] * td.tv3 * stresstu;
+    const R rd = R(P.gamma[g]) * vnom[g] * td.tv3;
+    R an_bb;
+    if constexpr (EXT) {
+      lf[g] = farq_c3_x(rad.topparu, s.ci[g], vmax, td.gstar, td.kc, td.ko, o2, rd, P, X->je[g],
+                        false, R(1), R(1));
+      if (sun_u) {
+        const SunShade<R>& ss = X->ss[0];
+        const Leaf<R> a = farq_c3_x(ss.a_sun, s.ci[g], vmax * ss.v_sun, td.gstar, td.kc, td.ko,
+                                    o2, rd * ss.v_sun, P, X->je[g], false, R(1), R(1));
+        const Leaf<R> b = farq_c3_x(ss.a_sha, s.ci[g], vmax * ss.v_sha, td.gstar, td.kc, td.ko,
+                                    o2, rd * ss.v_sha, P, X->je[g], false, R(1), R(1));
+        cr[g].ag = (a.ag + b.ag) * ss.inv;
+        cr[g].an = (a.an + b.an) * ss.inv;
+      }
+      ci0[g] = s.ci[g];
+      vmx[g] = vmax;
+      rdk[g] = rd;
+      an_bb = sun_u ? cr[g].an : lf[g].an;
+    } else {
+      lf[g] = farq_c3(rad.topparu, s.ci[g], vmax, td.gstar, td.kc, td.ko, o2, rd, P);
+      an_bb = lf[g].an;
+    }
+    ball_berry(an_bb, s.cs[g], s.gs[g], s.ci[g], co2, gb, rh12, slope(g, g), R(P.coefb[g]),
                stresstu, R(P.gsmin[g]), gmin, cimax, false);
   }
   R scale = canopy_scaling(rad.topparu, rad.termu6, rad.termu7, rad.scalcoefu, lai[1], sai[1],
-                           s.a10[0], s.a10[1], zw, eps);
+                           s.a10[0], s.a10[1], zw, eps, leaf_only);
 #pragma unroll
   for (int g = 0; g < 2; ++g) {
-    const R agc = lf[g].ag * scale;
-    const R anc = lf[g].an * scale;
+    R agc = lf[g].ag * scale;
+    R anc = lf[g].an * scale;
+    if constexpr (EXT) {
+      if (sun_u) {
+        agc = cr[g].ag;
+        anc = cr[g].an;
+      } else if (cap_u) {
+        const Leaf<R> c = farq_c3_x(rad.topparu, ci0[g], vmx[g], td.gstar, td.kc, td.ko, o2,
+                                    rdk[g], P, X->je[g], true, scale, X->fcap[0]);
+        agc = c.ag;
+        anc = c.an;
+      }
+    }
     const R csc = mx(gmin, co2 - anc / gb);
     const R gsc = mx(mx(R(P.gsmin[g]), R(P.coefb[g]) * stresstu),
-                     R(P.coefm[g]) * anc * rh12 / csc + R(P.coefb[g]) * stresstu);
+                     slope(g, g) * anc * rh12 / csc + R(P.coefb[g]) * stresstu);
     s.tcond[g] = total_cond(su, gsc);
     s.ag[g] = agc * (R(1) - fwetu);
     s.an[g] = anc * (R(1) - fwetu);
@@ -179,34 +339,85 @@ IBIS_HD void stomata(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R
   const R rh34 = mx(R(0.30), q34 / mx(qsat(esat(t34), psurf), eps));
   // slot k of the lower arrays -> Ball-Berry parameter group
   const int grp[3] = {2, 4, 3};
-  const R vnom_l[3] = {vls, vl3, vl4};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int g = grp[k];
     const bool c4 = (k == 2);
     s.ci[2 + k] = clip(s.ci[2 + k], c4 ? R(0) : gmin, cimax);
     const R tv = c4 ? td.tv4 : td.tv3;
-    const R vmax = vnom_l[k] * tv * stresstl;
-    const R rd = R(P.gamma[g]) * vnom_l[k] * tv;
-    lf[2 + k] = c4 ? farq_c4(rad.topparl, s.ci[2 + k], vmax, rd, P)
-                   : farq_c3(rad.topparl, s.ci[2 + k], vmax, td.gstar, td.kc, td.ko, o2, rd, P);
-    ball_berry(lf[2 + k].an, s.cs[2 + k], s.gs[2 + k], s.ci[2 + k], co2, gb, rh34,
-               R(P.coefm[g]), R(P.coefb[g]), stresstl, R(P.gsmin[g]), gmin, cimax, c4);
+    const R vmax = vnom[2 + k] * tv * stresstl;
+    const R rd = R(P.gamma[g]) * vnom[2 + k] * tv;
+    const R ci = s.ci[2 + k];
+    R an_bb;
+    if constexpr (EXT) {
+      const R jef = X->je[2 + k];
+      lf[2 + k] = c4 ? farq_c4_x(rad.topparl, ci, vmax, rd, P, jef, false, R(1), R(1))
+                     : farq_c3_x(rad.topparl, ci, vmax, td.gstar, td.kc, td.ko, o2, rd, P, jef,
+                                 false, R(1), R(1));
+      if (sun_l) {
+        const SunShade<R>& ss = X->ss[1];
+        const Leaf<R> a =
+            c4 ? farq_c4_x(ss.a_sun, ci, vmax * ss.v_sun, rd * ss.v_sun, P, jef, false, R(1), R(1))
+               : farq_c3_x(ss.a_sun, ci, vmax * ss.v_sun, td.gstar, td.kc, td.ko, o2,
+                           rd * ss.v_sun, P, jef, false, R(1), R(1));
+        const Leaf<R> b =
+            c4 ? farq_c4_x(ss.a_sha, ci, vmax * ss.v_sha, rd * ss.v_sha, P, jef, false, R(1), R(1))
+               : farq_c3_x(ss.a_sha, ci, vmax * ss.v_sha, td.gstar, td.kc, td.ko, o2,
+                           rd * ss.v_sha, P, jef, false, R(1), R(1));
+        cr[2 + k].ag = (a.ag + b.ag) * ss.inv;
+        cr[2 + k].an = (a.an + b.an) * ss.inv;
+      }
+      ci0[2 + k] = ci;
+      vmx[2 + k] = vmax;
+      rdk[2 + k] = rd;
+      an_bb = sun_l ? cr[2 + k].an : lf[2 + k].an;
+    } else {
+      lf[2 + k] = c4 ? farq_c4(rad.topparl, ci, vmax, rd, P)
+                     : farq_c3(rad.topparl, ci, vmax, td.gstar, td.kc, td.ko, o2, rd, P);
+      an_bb = lf[2 + k].an;
+    }
+    ball_berry(an_bb, s.cs[2 + k], s.gs[2 + k], s.ci[2 + k], co2, gb, rh34, slope(2 + k, g),
+               R(P.coefb[g]), stresstl, R(P.gsmin[g]), gmin, cimax, c4);
   }
   scale = canopy_scaling(rad.topparl, rad.terml6, rad.terml7, rad.scalcoefl, lai[0], sai[0],
-                         s.a10[2], s.a10[3], zw, eps);
+                         s.a10[2], s.a10[3], zw, eps, leaf_only);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int g = grp[k];
-    const R agc = lf[2 + k].ag * scale;
-    const R anc = lf[2 + k].an * scale;
+    R agc = lf[2 + k].ag * scale;
+    R anc = lf[2 + k].an * scale;
+    if constexpr (EXT) {
+      if (sun_l) {
+        agc = cr[2 + k].ag;
+        anc = cr[2 + k].an;
+      } else if (cap_l) {
+        const R jef = X->je[2 + k], fc = X->fcap[1];
+        const Leaf<R> c =
+            k == 2 ? farq_c4_x(rad.topparl, ci0[4], vmx[4], rdk[4], P, jef, true, scale, fc)
+                   : farq_c3_x(rad.topparl, ci0[2 + k], vmx[2 + k], td.gstar, td.kc, td.ko, o2,
+                               rdk[2 + k], P, jef, true, scale, fc);
+        agc = c.ag;
+        anc = c.an;
+      }
+    }
     const R csc = mx(k == 2 ? R(1.0e-8) : gmin, co2 - anc / gb);
     const R gsc = mx(mx(R(P.gsmin[g]), R(P.coefb[g]) * stresstl),
-                     R(P.coefm[g]) * anc * rh34 / csc + R(P.coefb[g]) * stresstl);
+                     slope(2 + k, g) * anc * rh34 / csc + R(P.coefb[g]) * stresstl);
     s.tcond[2 + k] = total_cond(sl, gsc);
     s.ag[2 + k] = agc * (R(1) - fwetl);
     s.an[2 + k] = anc * (R(1) - fwetl);
   }
+}
+
+// stomata on the F77 path: Vmax of the big leaves as given, group slopes, scaled leaf rates
+template <class R, class PT>
+IBIS_HD IBIS_INLINE void stomata(Stom<R>& s, R tu, R tl, R t12, R t34, R q12, R q34, R su, R sl,
+                                 R psurf, const RadOut<R>& rad, R fwetu, R fwetl,
+                                 const R lai[2], const R sai[2], R vub, R vuc, R vls, R vl3,
+                                 R vl4, R stresstu, R stresstl, R co2, R zw, const PT& P) {
+  const R vnom[5] = {vub, vuc, vls, vl3, vl4};
+  stomata_impl<R, PT, false>(s, tu, tl, t12, t34, q12, q34, su, sl, psurf, rad, fwetu, fwetl,
+                             lai, sai, vnom, stresstu, stresstl, co2, zw, P, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------

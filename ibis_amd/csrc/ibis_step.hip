@@ -24,6 +24,19 @@ int launch_f64(const IbisArena* in, const IbisArena* out, void* ws, const int8_t
                int64_t slot_pitch, const double* table, const IbisParams& P,
                const StepConst& K, int64_t ncell, int copy_in, hipStream_t st);
 
+int launch_ext_mixed(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                     int64_t slot_pitch, const double* tables, const IbisParams& P,
+                     const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                     hipStream_t st);
+int launch_ext_f32(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                   int64_t slot_pitch, const double* tables, const IbisParams& P,
+                   const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                   hipStream_t st);
+int launch_ext_f64(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                   int64_t slot_pitch, const double* tables, const IbisParams& P,
+                   const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                   hipStream_t st);
+
 static int64_t workspace_bytes(int64_t pitch, int prec) {
   switch (prec) {
     case IBIS_PREC_MIXED: return work_bytes(pitch, (int)sizeof(float), (int)sizeof(double));
@@ -46,44 +59,64 @@ int64_t ibis_step_workspace_bytes(int64_t pitch, int flags) {
   return ibis::workspace_bytes(pitch, prec);
 }
 
+}  // extern "C"
+
+namespace ibis {
+
+// Argument checks shared by ibis_step and ibis_step_ext, and the launch constants.
+static int check_step(const char* who, const IbisArena* in, const IbisArena* out,
+                      void* workspace, int64_t workspace_bytes, const int8_t* slot_type,
+                      int64_t slot_pitch, const IbisParams* params, double dtime,
+                      double nytimes, int64_t ncell, int flags, StepConst& K) {
+  if (!in || !out || !params || !in->planes || !out->planes)
+    return fail(IBIS_E_ARG, "%s: null pointer argument", who);
+  if (ncell < 0) return fail(IBIS_E_ARG, "%s: negative ncell", who);
+  if (in->pitch < ncell || out->pitch != in->pitch)
+    return fail(IBIS_E_ARG, "%s: pitch must be >= ncell and equal for in and out", who);
+  if (slot_type && slot_pitch < ncell)
+    return fail(IBIS_E_ARG, "%s: slot_pitch must be >= ncell", who);
+  if (!(dtime > 0.0)) return fail(IBIS_E_ARG, "%s: dtime must be positive", who);
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~(IBIS_PREC_MASK | IBIS_FLAG_BEDROCK | IBIS_FLAG_COPY_IN)) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "%s: unknown flag bits or precision", who);
+  if (!workspace || workspace_bytes < ibis::workspace_bytes(in->pitch, prec))
+    return fail(IBIS_E_ARG, "%s: workspace missing or smaller than "
+                            "ibis_step_workspace_bytes(pitch, flags)", who);
+  if (((uintptr_t)workspace & 255u) != 0)
+    return fail(IBIS_E_ARG, "%s: workspace must be 256-byte aligned", who);
+  const IbisParams& P = *params;
+  if (flags & IBIS_FLAG_BEDROCK) {
+    for (int k = 0; k < IBIS_NBEDROCK; ++k)
+      if (!(P.bed_hsoi[k] > 0.0))
+        return fail(IBIS_E_BEDROCK, "%s: bedrock layer thicknesses must be positive", who);
+  }
+  K.dtime = dtime;
+  K.tfac = 1.0 / pow(0.999, P.cappa);
+  K.zweight = exp(-1.0 / (10.0 * 86400.0 / dtime));
+  K.nytimes_new = nytimes + 1.0;
+  K.bedrock = (flags & IBIS_FLAG_BEDROCK) ? 1 : 0;
+  return IBIS_OK;
+}
+
+}  // namespace ibis
+
+extern "C" {
+
 int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
               int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
               const double* type_table, const IbisParams* params, double dtime, double nytimes,
               int64_t ncell, int flags, void* hip_stream) {
   using namespace ibis;
   err_buf()[0] = 0;
-  if (!in || !out || !params || !type_table || !in->planes || !out->planes)
-    return fail(IBIS_E_ARG, "ibis_step: null pointer argument");
-  if (ncell < 0) return fail(IBIS_E_ARG, "ibis_step: negative ncell");
-  if (in->pitch < ncell || out->pitch != in->pitch)
-    return fail(IBIS_E_ARG, "ibis_step: pitch must be >= ncell and equal for in and out");
-  if (slot_type && slot_pitch < ncell)
-    return fail(IBIS_E_ARG, "ibis_step: slot_pitch must be >= ncell");
-  if (!(dtime > 0.0)) return fail(IBIS_E_ARG, "ibis_step: dtime must be positive");
-  const int prec = flags & IBIS_PREC_MASK;
-  if ((flags & ~(IBIS_PREC_MASK | IBIS_FLAG_BEDROCK | IBIS_FLAG_COPY_IN)) != 0 || prec > 2)
-    return fail(IBIS_E_FLAGS, "ibis_step: unknown flag bits or precision");
-  if (!workspace || workspace_bytes < ibis::workspace_bytes(in->pitch, prec))
-    return fail(IBIS_E_ARG, "ibis_step: workspace missing or smaller than "
-                            "ibis_step_workspace_bytes(pitch, flags)");
-  if (((uintptr_t)workspace & 255u) != 0)
-    return fail(IBIS_E_ARG, "ibis_step: workspace must be 256-byte aligned");
-  const IbisParams& P = *params;
-  if (flags & IBIS_FLAG_BEDROCK) {
-    for (int k = 0; k < IBIS_NBEDROCK; ++k)
-      if (!(P.bed_hsoi[k] > 0.0))
-        return fail(IBIS_E_BEDROCK, "ibis_step: bedrock layer thicknesses must be positive");
-  }
-  if (ncell == 0) return IBIS_OK;
+  if (!type_table) return fail(IBIS_E_ARG, "ibis_step: null pointer argument");
   StepConst K;
-  K.dtime = dtime;
-  K.tfac = 1.0 / pow(0.999, P.cappa);
-  K.zweight = exp(-1.0 / (10.0 * 86400.0 / dtime));
-  K.nytimes_new = nytimes + 1.0;
-  K.bedrock = (flags & IBIS_FLAG_BEDROCK) ? 1 : 0;
+  const int rc = check_step("ibis_step", in, out, workspace, workspace_bytes, slot_type,
+                            slot_pitch, params, dtime, nytimes, ncell, flags, K);
+  if (rc != IBIS_OK || ncell == 0) return rc;
+  const IbisParams& P = *params;
   const int copy_in = (flags & IBIS_FLAG_COPY_IN) ? 1 : 0;
   hipStream_t st = (hipStream_t)hip_stream;
-  switch (prec) {
+  switch (flags & IBIS_PREC_MASK) {
     case IBIS_PREC_MIXED:
       return launch_mixed(in, out, workspace, slot_type, slot_pitch, type_table, P, K, ncell,
                           copy_in, st);
@@ -93,6 +126,44 @@ int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
     default:
       return launch_f64(in, out, workspace, slot_type, slot_pitch, type_table, P, K, ncell,
                         copy_in, st);
+  }
+}
+
+int ibis_step_ext(const IbisArena* in, const IbisArena* out, void* workspace,
+                  int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
+                  const IbisParams* params, const IbisStomataExt* ext, double dtime,
+                  double nytimes, int64_t ncell, int flags, void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (!ext || !ext->type_tables) return fail(IBIS_E_ARG, "ibis_step_ext: null pointer argument");
+  if ((ext->switches & ~IBIS_SW_ALL) != 0)
+    return fail(IBIS_E_FLAGS, "ibis_step_ext: unknown switch bits");
+  if ((ext->acclim_mask & ~((1 << IBIS_NPFT) - 1)) != 0)
+    return fail(IBIS_E_ARG, "ibis_step_ext: acclim_mask names a plant type beyond %d",
+                IBIS_NPFT - 1);
+  const bool acclim = (ext->switches & IBIS_SW_ACCLIM) != 0;
+  if (acclim && !ext->acclim_x)
+    return fail(IBIS_E_ARG, "ibis_step_ext: IBIS_SW_ACCLIM needs the acclimation state");
+  if (acclim && (ext->switches & IBIS_SW_FROST) && !ext->td)
+    return fail(IBIS_E_ARG, "ibis_step_ext: IBIS_SW_FROST needs the daily mean temperature");
+  StepConst K;
+  const int rc = check_step("ibis_step_ext", in, out, workspace, workspace_bytes, slot_type,
+                            slot_pitch, params, dtime, nytimes, ncell, flags, K);
+  if (rc != IBIS_OK || ncell == 0) return rc;
+  const StepExt E = step_ext_from(*ext);
+  const IbisParams& P = *params;
+  const int copy_in = (flags & IBIS_FLAG_COPY_IN) ? 1 : 0;
+  hipStream_t st = (hipStream_t)hip_stream;
+  switch (flags & IBIS_PREC_MASK) {
+    case IBIS_PREC_MIXED:
+      return launch_ext_mixed(in, out, workspace, slot_type, slot_pitch, ext->type_tables, P, K,
+                              E, ncell, copy_in, st);
+    case IBIS_PREC_F32:
+      return launch_ext_f32(in, out, workspace, slot_type, slot_pitch, ext->type_tables, P, K,
+                            E, ncell, copy_in, st);
+    default:
+      return launch_ext_f64(in, out, workspace, slot_type, slot_pitch, ext->type_tables, P, K,
+                            E, ncell, copy_in, st);
   }
 }
 

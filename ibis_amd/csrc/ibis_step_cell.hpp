@@ -36,6 +36,38 @@ template <class S> struct Cell {
   IBIS_HD IBIS_INLINE S lo(int plane) const { return out[(int64_t)plane * pitch + i]; }
 };
 
+// The optional StomataParams switches of a launch and the per-cell inputs they read
+// (IbisStomataExt without the tables, which the canopy kernel stages in LDS).  The per-cell
+// pointers hold ncell elements of the arena's type.
+struct StepExt {
+  int sw, acclim_mask;
+  double cap_kn, acclim_s0, acclim_smax, frost_t0, frost_t1;
+  const void* acclim_x;
+  const void* td;
+  const void* stressfac;
+  IBIS_HD IBIS_INLINE bool on(int bits) const { return (sw & bits) != 0; }
+};
+
+// The launch's StepExt from the C ABI's IbisStomataExt.  acclim_light, acclim_per_pft and the
+// frost ramp act only through the acclimation factor (loops.py:291-310): without
+// IBIS_SW_ACCLIM they are off.
+inline StepExt step_ext_from(const IbisStomataExt& x) {
+  StepExt E;
+  const bool acclim = (x.switches & IBIS_SW_ACCLIM) != 0;
+  E.sw = acclim ? x.switches
+                : (x.switches & ~(IBIS_SW_ACCLIM_PER_PFT | IBIS_SW_ACCLIM_LIGHT | IBIS_SW_FROST));
+  E.acclim_mask = x.acclim_mask;
+  E.cap_kn = x.cap_kn;
+  E.acclim_s0 = x.acclim_s0;
+  E.acclim_smax = x.acclim_smax;
+  E.frost_t0 = x.frost_t0;
+  E.frost_t1 = x.frost_t1;
+  E.acclim_x = x.acclim_x;
+  E.td = x.td;
+  E.stressfac = x.stressfac;
+  return E;
+}
+
 // Per-cell trace points for host-side debugging builds; empty in every product build.
 #ifndef IBIS_TRACE
 #define IBIS_TRACE(cell, tag, value) ((void)0)
@@ -63,7 +95,8 @@ enum WorkRPlane {
   WR_FVAPLW,
   WR_FVAPA,
   WR_FSENA,
-  WR_N
+  WR_DBEAM,                    // 2: dbeamu dbeaml (written only for the sunlit/shaded canopies)
+  WR_N = WR_DBEAM + 2
 };
 
 template <class R, class D> struct Work {
@@ -100,9 +133,9 @@ template <class R> IBIS_HD IBIS_INLINE void aero_load(Aero<R>& a, const R* v) {
 #define ST(P, v) c.st((P), S(v))
 #define LO(P) R(c.lo(P))
 
-template <class S, class R, class D, class PT>
-IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
-                          const StepConst& K) {
+template <class S, class R, class D, class PT, bool EXT>
+IBIS_HD void surface_cell_x(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
+                            const StepConst& K, const StepExt* E) {
   IBIS_TRACE(c.i, "cell", 0);
   const R eps = R(P.epsilon), tmelt = R(P.tmelt);
   const R dtime = R(K.dtime);
@@ -206,6 +239,7 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
     }
     rad.termu6 = R(rd.termu6); rad.termu7 = R(rd.termu7);
     rad.terml6 = R(rd.terml6); rad.terml7 = R(rd.terml7);
+    rad.dbeamu = R(rd.dbeamu); rad.dbeaml = R(rd.dbeaml);
   }
   IrOut<D> ir;
   irrad(ir, D(fira), laiD, saiD, D(fu), D(fl), D(fi0), D(tu0), D(ts0), D(tl0), D(tg0), D(ti0), P);
@@ -256,7 +290,10 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
   // drystress (stomata.py:515-562)
   R stressl[NS], stressu[NS], stresstl, stresstu;
   {
-    const R sf = R(P.stressfac);
+    R sf = R(P.stressfac);
+    if constexpr (EXT) {
+      if (E->stressfac) sf = R(((const S*)E->stressfac)[c.i]);
+    }
     const R znorm = R(1) - r_exp(sf);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
@@ -297,6 +334,12 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
   wk.str(WR_TERM + 1, rad.termu7);
   wk.str(WR_TERM + 2, rad.terml6);
   wk.str(WR_TERM + 3, rad.terml7);
+  if constexpr (EXT) {
+    if (E->on(IBIS_SW_SUNSHADE_U | IBIS_SW_SUNSHADE_L)) {
+      wk.str(WR_DBEAM + 0, rad.dbeamu);
+      wk.str(WR_DBEAM + 1, rad.dbeaml);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < 6; ++j) wk.str(WR_W + j, cas.w[j]);
   wk.str(WR_PFLUX + 0, cas.pfluxu);
@@ -312,9 +355,93 @@ IBIS_HD void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
   ST(IBIS_P_HYDROLOGY_TSNOWL, cas.tsnowl);
 }
 
-template <class S, class R, class D, class PT>
-IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
-                         const PT& P, const StepConst& K) {
+// Per-cell inputs of the five big leaves under the optional switches (stomata.py:732-1080,
+// loops.py:291-310): the acclimation factor of each plant type, the nominal Vmax by existence
+// priority or composition, the light-rate factors, the slopes, the sunlit/shaded partitions and
+// the capacity integrals.  xrow: per-slot coefm, acclim_s0, acclim_smax.
+template <class S, class R, class PT>
+IBIS_HD void big_leaves(StomExt<R>& X, const Cell<S>& c, const StepExt& E,
+                        const R vrow[IBIS_NPFT], const R xrow[3][IBIS_NPFT],
+                        const RadOut<R>& rad, const R lai[2], const R sai[2], const PT& P) {
+  const R eps = R(P.epsilon);
+  R frac[IBIS_NPFT], vp[IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    frac[j] = LD(IBIS_P_VEGETATION_FRAC + j);
+    vp[j] = vrow[j];
+  }
+  const bool acclim = E.on(IBIS_SW_ACCLIM);
+  if (acclim) {
+    const R ax = R(((const S*)E.acclim_x)[c.i]);
+    R ff = R(1);
+    if (E.on(IBIS_SW_FROST)) {
+      const R tdc = R(((const S*)E.td)[c.i]) - R(273.16);
+      ff = clip((tdc - R(E.frost_t0)) / (R(E.frost_t1) - R(E.frost_t0)), R(0), R(1));
+    }
+#pragma unroll
+    for (int j = 0; j < IBIS_NPFT; ++j) {
+      if (E.acclim_mask & (1 << j)) {
+        const R s0 = E.on(IBIS_SW_ACCLIM_PER_PFT) ? xrow[1][j] : R(E.acclim_s0);
+        const R smax = E.on(IBIS_SW_ACCLIM_PER_PFT) ? xrow[2][j] : R(E.acclim_smax);
+        vp[j] = vrow[j] * (clip((ax - s0) / smax, R(0), R(1)) * ff);
+      }
+    }
+  }
+  const R ex0 = LD(IBIS_P_VEGETATION_EXIST + 0), ex2 = LD(IBIS_P_VEGETATION_EXIST + 2);
+  const R ex3 = LD(IBIS_P_VEGETATION_EXIST + 3);
+  const bool mix = E.on(IBIS_SW_VMAX_MIX);
+  // priority picks (stomata.py:598-650), then the composition means
+  auto select = [&](const R v[IBIS_NPFT], R out[5]) {
+    out[0] = ex0 > R(0.5) ? v[0] : (ex2 > R(0.5) ? v[2] : v[4]);
+    out[1] = ex3 > R(0.5) ? v[3] : v[5];
+    out[2] = v[8];
+    if (mix) {
+      out[0] = mix_pft(frac, v, kBroadleaf, out[0]);
+      out[1] = mix_pft(frac, v, kConifer, out[1]);
+      out[2] = mix_pft(frac, v, kShrub, out[2]);
+    }
+    out[3] = v[11];
+    out[4] = v[10];
+  };
+  select(vp, X.vnom);
+  if (acclim && E.on(IBIS_SW_ACCLIM_LIGHT)) {
+    R raw[5];
+    select(vrow, raw);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) X.je[k] = X.vnom[k] / mx(raw[k], eps);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) X.je[k] = R(1);
+  }
+  if (E.on(IBIS_SW_COEFM_MIX)) {
+    X.cm[0] = mix_pft(frac, xrow[0], kBroadleaf, R(P.coefm[0]));
+    X.cm[1] = mix_pft(frac, xrow[0], kConifer, R(P.coefm[1]));
+    X.cm[2] = mix_pft(frac, xrow[0], kShrub, R(P.coefm[2]));
+    X.cm[3] = xrow[0][11];
+    X.cm[4] = xrow[0][10];
+  } else {
+    X.cm[0] = R(P.coefm[0]);
+    X.cm[1] = R(P.coefm[1]);
+    X.cm[2] = R(P.coefm[2]);
+    X.cm[3] = R(P.coefm[4]);
+    X.cm[4] = R(P.coefm[3]);
+  }
+  const R kn = R(E.cap_kn);
+  if (E.on(IBIS_SW_SUNSHADE_U))
+    X.ss[0] = sunshade_canopy(rad.scalcoefu, rad.dbeamu, rad.termu6, rad.termu7, lai[1], sai[1],
+                              kn, eps);
+  if (E.on(IBIS_SW_SUNSHADE_L))
+    X.ss[1] = sunshade_canopy(rad.scalcoefl, rad.dbeaml, rad.terml6, rad.terml7, lai[0], sai[0],
+                              kn, eps);
+  X.fcap[0] = capacity_integral(lai[1], kn);
+  X.fcap[1] = capacity_integral(lai[0], kn);
+  X.sw = E.sw;
+}
+
+template <class S, class R, class D, class PT, bool EXT>
+IBIS_HD void canopy_cell_x(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
+                           const PT& P, const StepConst& K, const StepExt* E,
+                           const R (*xrow)[IBIS_NPFT]) {
   const R eps = R(P.epsilon);
   const D dtD = D(K.dtime), tmeltD = D(P.tmelt);
 
@@ -360,6 +487,13 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
   rad.termu7 = wk.ldr(WR_TERM + 1);
   rad.terml6 = wk.ldr(WR_TERM + 2);
   rad.terml7 = wk.ldr(WR_TERM + 3);
+  rad.dbeamu = rad.dbeaml = R(0);
+  if constexpr (EXT) {
+    if (E->on(IBIS_SW_SUNSHADE_U | IBIS_SW_SUNSHADE_L)) {
+      rad.dbeamu = wk.ldr(WR_DBEAM + 0);
+      rad.dbeaml = wk.ldr(WR_DBEAM + 1);
+    }
+  }
   IrOut<D> ir;
   ir.firu = wk.ldd(WD_IR + 0);
   ir.firs = wk.ldd(WD_IR + 1);
@@ -425,13 +559,23 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
   const R wlgx = wpud + rwp * (R(1) - wisoi0) * wsoi0 * hsoi0;
   const R wigx = wipud + rwp * wisoi0 * hsoi0;
 
-  // Vmax of the big leaves by existence priority (stomata.py:598-650)
-  R ex[5];
-  ex[0] = LD(IBIS_P_VEGETATION_EXIST + 0);
-  ex[1] = LD(IBIS_P_VEGETATION_EXIST + 2);
-  ex[2] = LD(IBIS_P_VEGETATION_EXIST + 3);
-  const R vub = ex[0] > R(0.5) ? vrow[0] : (ex[1] > R(0.5) ? vrow[2] : vrow[4]);
-  const R vuc = ex[2] > R(0.5) ? vrow[3] : vrow[5];
+  // Vmax of the big leaves by existence priority (stomata.py:598-650), or the inputs of the
+  // optional switches
+  R vnom[5];
+  StomExt<R> X;
+  if constexpr (EXT) {
+    big_leaves(X, c, *E, vrow, xrow, rad, lai, sai, P);
+  } else {
+    R ex[5];
+    ex[0] = LD(IBIS_P_VEGETATION_EXIST + 0);
+    ex[1] = LD(IBIS_P_VEGETATION_EXIST + 2);
+    ex[2] = LD(IBIS_P_VEGETATION_EXIST + 3);
+    vnom[0] = ex[0] > R(0.5) ? vrow[0] : (ex[1] > R(0.5) ? vrow[2] : vrow[4]);
+    vnom[1] = ex[2] > R(0.5) ? vrow[3] : vrow[5];
+    vnom[2] = vrow[8];
+    vnom[3] = vrow[11];
+    vnom[4] = vrow[10];
+  }
 
   Stom<R> sm;
   sm.ci[0] = LD(IBIS_P_CANOPY_CI_UPPER); sm.ci[1] = LD(IBIS_P_CANOPY_CI_UPPER + 1);
@@ -464,9 +608,9 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
     Turcof<D> tc;
     turcof(tc, aero, D(K.tfac), taD, qaD, uaD, cur[3], cur[4], cur[5], cur[6], dtD, first, u2c,
            aloguc, aloglc, straml, strahl, stramu, strahu, P);
-    stomata(sm, R(cur[0]), R(cur[2]), R(cur[3]), R(cur[4]), R(cur[5]), R(cur[6]), R(tc.su), R(tc.sl),
-            psurf, rad, fw2.fwetu, fw2.fwetl, lai, sai, vub, vuc, vrow[8], vrow[11], vrow[10],
-            stresstu, stresstl, co2, zw, P);
+    stomata_impl<R, PT, EXT>(sm, R(cur[0]), R(cur[2]), R(cur[3]), R(cur[4]), R(cur[5]),
+                             R(cur[6]), R(tc.su), R(tc.sl), psurf, rad, fw2.fwetu, fw2.fwetl,
+                             lai, sai, EXT ? X.vnom : vnom, stresstu, stresstl, co2, zw, P, &X);
     if (!first) {
       wp[0] = impexp(cur[0], D(F.chux), D(F.w[0]), D(F.w[3]), wp[0], P);
       wp[1] = impexp(cur[1], D(F.chsx), D(F.w[1]), D(F.w[4]), wp[1], P);
@@ -541,6 +685,18 @@ IBIS_HD void canopy_cell(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IB
   wk.str(WR_FSENA, R(tv.fsena));
   wk.std_(WD_FVAPUT, tv.fvaput);
   wk.std_(WD_FVAPLT, tv.fvaplt);
+}
+
+// The F77 path of the surface and canopy phases (every optional switch off).
+template <class S, class R, class D, class PT>
+IBIS_HD IBIS_INLINE void surface_cell(const Cell<S>& c, const Work<R, D>& wk, const PT& P,
+                                      const StepConst& K) {
+  surface_cell_x<S, R, D, PT, false>(c, wk, P, K, nullptr);
+}
+template <class S, class R, class D, class PT>
+IBIS_HD IBIS_INLINE void canopy_cell(const Cell<S>& c, const Work<R, D>& wk,
+                                     const R vrow[IBIS_NPFT], const PT& P, const StepConst& K) {
+  canopy_cell_x<S, R, D, PT, false>(c, wk, vrow, P, K, nullptr, nullptr);
 }
 
 template <class S, class R, class D, class PT>

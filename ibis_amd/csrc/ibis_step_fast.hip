@@ -19,4 +19,20 @@ int launch_f32(const IbisArena* in, const IbisArena* out, void* ws, const int8_t
   return launch<float, float, float>(in, out, ws, slot, slot_pitch, table, P, K, ncell, copy_in, st);
 }
 
+int launch_ext_mixed(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                     int64_t slot_pitch, const double* tables, const IbisParams& P,
+                     const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                     hipStream_t st) {
+  return launch_ext<float, float, double>(in, out, ws, slot, slot_pitch, tables, P, K, E, ncell,
+                                          copy_in, st);
+}
+
+int launch_ext_f32(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                   int64_t slot_pitch, const double* tables, const IbisParams& P,
+                   const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                   hipStream_t st) {
+  return launch_ext<float, float, float>(in, out, ws, slot, slot_pitch, tables, P, K, E, ncell,
+                                         copy_in, st);
+}
+
 }  // namespace ibis

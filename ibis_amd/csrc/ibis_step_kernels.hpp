@@ -76,6 +76,75 @@ __global__ __launch_bounds__(kBlock, kWavesGround) void ground_kernel(
   ground_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), P, K);
 }
 
+// The step with the optional StomataParams switches (ibis_step_ext): the surface phase reads
+// a per-cell stressfac and hands the unscattered beam terms over; the canopy phase stages the
+// IBIS_NTABLE per-slot tables in LDS and builds the big leaves' inputs per cell.  The ground
+// phase is the plain one.
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock, kWavesSurface) void surface_kernel_ext(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, IbisParams P, StepConst K,
+    int copy_in, StepExt E) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  if (copy_in) {
+#define X(F, W, OFF, ROLE)                                                      \
+  if ((ROLE) == IBIS_ROLE_IN) {                                                 \
+    for (int j = 0; j < (W); ++j)                                               \
+      out[(int64_t)((OFF) + j) * pitch + i] = in[(int64_t)((OFF) + j) * pitch + i]; \
+  }
+    IBIS_FIELD_TABLE
+#undef X
+  }
+  const Cell<S> c{in, out, pitch, i};
+  surface_cell_x<S, R, D, IbisParams, true>(c, work_at<R, D>(ws, pitch, i), P, K, &E);
+}
+
+template <class S, class R, class D>
+__global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel_ext(
+    const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, const int8_t* slot,
+    int64_t slot_pitch, const double* tables, IbisParams P, StepConst K, StepExt E) {
+  __shared__ R trow[IBIS_NTABLE][IBIS_TYPE_ROWS];
+  for (int t = threadIdx.x; t < IBIS_NTABLE * IBIS_TYPE_ROWS; t += blockDim.x)
+    trow[t / IBIS_TYPE_ROWS][t % IBIS_TYPE_ROWS] = R(tables[t]);
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ncell) return;
+  R vrow[IBIS_NPFT], xrow[3][IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) {
+    const int s = slot ? (int)slot[(int64_t)j * slot_pitch + i] : -1;
+    const int row = (s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j;
+    vrow[j] = trow[0][row];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) xrow[q][j] = trow[1 + q][row];
+  }
+  const Cell<S> c{in, out, pitch, i};
+  canopy_cell_x<S, R, D, IbisParams, true>(c, work_at<R, D>(ws, pitch, i), vrow, P, K, &E, xrow);
+}
+
+template <class S, class R, class D>
+inline int launch_ext(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
+                      int64_t slot_pitch, const double* tables, const IbisParams& P,
+                      const StepConst& K, const StepExt& E, int64_t ncell, int copy_in,
+                      hipStream_t st) {
+  const dim3 grid((unsigned)((ncell + kBlock - 1) / kBlock)), block(kBlock);
+  const S* pin = (const S*)in->planes;
+  S* pout = (S*)out->planes;
+  const int64_t pitch = in->pitch;
+  hipLaunchKernelGGL((surface_kernel_ext<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch,
+                     ncell, P, K, copy_in, E);
+  hipLaunchKernelGGL((canopy_kernel_ext<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch,
+                     ncell, slot, slot_pitch, tables, P, K, E);
+  hipLaunchKernelGGL((ground_kernel<S, R, D>), grid, block, 0, st, pin, pout, ws, pitch, ncell,
+                     P, K);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(err_buf(), 512, "ibis_step_ext: launch failed: %s", hipGetErrorString(e));
+    return IBIS_E_LAUNCH;
+  }
+  return IBIS_OK;
+}
+
 template <class S, class R, class D>
 inline int launch(const IbisArena* in, const IbisArena* out, void* ws, const int8_t* slot,
                   int64_t slot_pitch, const double* table, const IbisParams& P,

@@ -51,6 +51,12 @@ DEFAULT_VMAX = (65e-6, 65e-6, 40e-6, 30e-6, 30e-6, 25e-6, 30e-6, 30e-6, 27.5e-6,
                 15e-6, 25e-6)
 
 
+#: the boolean StomataParams switches the hourly step reads (stomata.py:93-161)
+SWITCH_NAMES = ("vmax_mix", "scale_leaf_only", "acclim_on", "cap_scale_u", "cap_scale_l",
+                "sunshade_u", "sunshade_l", "coefm_mix", "acclim_per_pft", "acclim_light",
+                "frost_on")
+
+
 @dataclass
 class StomataParams:
     tau15: float = 4500.0
@@ -70,7 +76,37 @@ class StomataParams:
     coefb: np.ndarray = field(default_factory=lambda: np.array([0.010, 0.010, 0.010, 0.040, 0.010]))
     gsmin: np.ndarray = field(default_factory=lambda: np.array([1e-5] * 5))
     vmax_pft: np.ndarray = field(default_factory=lambda: np.array(DEFAULT_VMAX))
+    #: scalar, or (npoi,) per cell (the per-zone value of python-port/ibis/io/pft_types.py:218-230)
     stressfac: float = -5.0
+    # optional switches (python-port/ibis/physics/stomata.py:93-161); all off is the F77 path
+    vmax_mix: bool = False
+    scale_leaf_only: bool = False
+    acclim_on: bool = False
+    acclim_pfts: tuple = (3, 5, 8)
+    acclim_tau: float = 10.93
+    acclim_s0: float = -3.063
+    acclim_smax: float = 17.72
+    cap_scale_u: bool = False
+    cap_scale_l: bool = False
+    cap_kn: float = 0.15
+    sunshade_u: bool = False
+    sunshade_l: bool = False
+    coefm_mix: bool = False
+    coefm_pft: np.ndarray = field(default_factory=lambda: np.array(
+        [10.0, 10.0, 10.0, 6.0, 10.0, 6.0, 10.0, 6.0, 9.0, 9.0, 4.0, 9.0]))
+    acclim_per_pft: bool = False
+    acclim_light: bool = False
+    frost_on: bool = False
+    frost_t0: float = -5.0
+    frost_t1: float = 5.0
+    gpp_split_by_capacity: bool = False
+    acclim_s0_pft: np.ndarray = field(default_factory=lambda: np.full(12, -3.063))
+    acclim_smax_pft: np.ndarray = field(default_factory=lambda: np.full(12, 17.72))
+
+    def switches_on(self) -> bool:
+        """Whether any optional switch (or a per-cell stressfac) leaves the F77 path."""
+        return (any(bool(getattr(self, k)) for k in SWITCH_NAMES)
+                or np.ndim(self.stressfac) != 0)
 
 
 @dataclass
@@ -146,9 +182,12 @@ class SimParams:
     rhosoi_val: float = 2636.5
     csoi_mineral_val: float = 880.5
     bedrock: BedrockParams | None = None
-    #: per-type overrides of Vmax {type index 0..39: value}, IBIS 3.0 zone types
-    #: (python-port/ibis/io/pft_types.py:170-189)
+    #: per-type overrides {type index 0..39: value} of Vmax, the stomatal slope and the
+    #: acclimation constants, IBIS 3.0 zone types (python-port/ibis/io/pft_types.py:170-212)
     vmax_type_overrides: dict = field(default_factory=dict)
+    coefm_type_overrides: dict = field(default_factory=dict)
+    acclim_s0_type_overrides: dict = field(default_factory=dict)
+    acclim_smax_type_overrides: dict = field(default_factory=dict)
 
 
 def default_sim_params(bedrock: bool = True) -> SimParams:
@@ -183,3 +222,110 @@ def root_fractions(hsoi=SOIL_LAYERS, beta_lower: float = 0.950, beta_upper: floa
         prev = np.concatenate([[0.0], cum[:-1]])
         out[:, col] = (cum - prev) / (1.0 - beta ** depth[-1])
     return out
+
+
+# ---- the reference's parameter files ---------------------------------------------------
+
+#: IBIS PFT names, long and short spellings (python-port/ibis/io/physics_params.py:39-60)
+PFT_NAMES = ("tropical_broadleaf_evergreen", "tropical_broadleaf_deciduous",
+             "warm_temperate_broadleaf_evergreen", "temperate_conifer_evergreen",
+             "temperate_broadleaf_deciduous", "boreal_conifer_evergreen",
+             "boreal_broadleaf_deciduous", "boreal_conifer_deciduous", "evergreen_shrub",
+             "deciduous_shrub", "c4_grass", "c3_grass")
+PFT_SHORT_NAMES = ("trop_bl_ever", "trop_bl_dec", "warm_bl_ever", "temp_con_ever",
+                   "temp_bl_dec", "boreal_con_ever", "boreal_bl_dec", "boreal_con_dec",
+                   "ever_shrub", "dec_shrub", "c4_grass", "c3_grass")
+_PFT_INDEX = {**{n: i for i, n in enumerate(PFT_NAMES)},
+              **{n: i for i, n in enumerate(PFT_SHORT_NAMES)}}
+#: Ball-Berry groups of the yaml 'ball_berry' table, in IbisParams order
+BALL_BERRY_GROUPS = ("broadleaf", "conifer", "shrub", "c4_grass", "c3_grass")
+#: PFT -> Ball-Berry group (python-port/ibis/physics/stomata.py:654)
+PFT_SLOPE_GROUP = (0, 0, 0, 1, 0, 1, 0, 1, 2, 2, 3, 4)
+#: the 40 zone-type ids in type-index order (python-port/ibis/io/pft_types.py:50-92)
+TYPE_IDS = ("T01", "T02", "T03", "T04", "T05", "T06", "T07", "T08", "T09", "T10", "T11", "T12",
+            "T13", "T14", "T15", "T16", "T17", "T18", "T19", "T21", "S01", "S02", "S03", "S04",
+            "S05", "S06", "S07", "G01", "G02", "G03", "G04", "G05", "G06", "G07", "G08", "W01",
+            "W02", "W03", "C01", "C02")
+#: yaml per-type keys the hourly step reads -> SimParams override dict
+TYPE_KEYS = {"vmax": "vmax_type_overrides", "coefm": "coefm_type_overrides",
+             "acclim_s0": "acclim_s0_type_overrides",
+             "acclim_smax": "acclim_smax_type_overrides"}
+
+
+def _pft_vector(table, defaults):
+    out = [float(x) for x in defaults]
+    for name, v in (table or {}).items():
+        if name not in _PFT_INDEX:
+            raise KeyError(f"unknown PFT name {name!r}; use one of {PFT_NAMES} or {PFT_SHORT_NAMES}")
+        out[_PFT_INDEX[name]] = float(v)
+    return np.array(out)
+
+
+def sim_params_from_dict(d: dict, bedrock: bool = True) -> SimParams:
+    """SimParams from a parsed parameter file of the reference (the sections the hourly step
+    reads: stomata, veg_rad, canopy, interception, soil, the hourly fire thresholds and the
+    per-type vmax / coefm / acclim_s0 / acclim_smax of 'types'), with the reference's defaults
+    for absent keys (python-port/ibis/io/physics_params.py:129-247, :442-493;
+    io/pft_types.py:246-258).  Per-type stressfac and other per-zone scalars need the zone map
+    and are given per cell by the caller (``stomata_params.stressfac`` as an (npoi,) array)."""
+    d = d or {}
+    s = d.get("stomata", {}) or {}
+    sp = StomataParams()
+    for k in ("tau15", "kc15", "ko15", "cimax", "o2conc", "alpha3", "theta3", "beta3",
+              "alpha4", "theta4", "beta4", "stressfac", "acclim_tau", "acclim_s0",
+              "acclim_smax", "cap_kn", "frost_t0", "frost_t1"):
+        if k in s:
+            setattr(sp, k, float(s[k]))
+    for k in SWITCH_NAMES + ("gpp_split_by_capacity",):
+        setattr(sp, k, bool(s.get(k, False)))
+    if "acclim_pfts" in s:
+        sp.acclim_pfts = tuple(int(k) for k in s["acclim_pfts"])
+    bb = s.get("ball_berry", {}) or {}
+    for name in ("gamma", "coefm", "coefb", "gsmin"):
+        base = getattr(sp, name)
+        setattr(sp, name, np.array([float((bb.get(g, {}) or {}).get(name, base[i]))
+                                    for i, g in enumerate(BALL_BERRY_GROUPS)]))
+    vm = s.get("vmax", {}) or {}
+    sp.vmax_pft = np.array([float(vm.get(n, DEFAULT_VMAX[i])) for i, n in enumerate(PFT_NAMES)])
+    sp.coefm_pft = _pft_vector(s.get("coefm_pft"),
+                               [sp.coefm[PFT_SLOPE_GROUP[i]] for i in range(12)])
+    sp.acclim_s0_pft = _pft_vector(s.get("acclim_s0_pft"), [sp.acclim_s0] * 12)
+    sp.acclim_smax_pft = _pft_vector(s.get("acclim_smax_pft"), [sp.acclim_smax] * 12)
+
+    sim = default_sim_params(bedrock)
+    sim.stomata_params = sp
+    vr = d.get("veg_rad", {}) or {}
+    for k in ("rhoveg", "tauveg", "oriev", "orieh"):
+        if k in vr:
+            setattr(sim.veg_rad, k, np.asarray(vr[k], dtype=np.float32).astype(np.float64))
+    cp = d.get("canopy", {}) or {}
+    for k, v in cp.items():
+        if hasattr(sim.canopy, k):
+            setattr(sim.canopy, k, tuple(float(x) for x in v) if isinstance(v, list) else float(v))
+    ip = d.get("interception", {}) or {}
+    for k, v in ip.items():
+        if hasattr(sim.interception, k):
+            setattr(sim.interception, k, float(v))
+    so = d.get("soil", {}) or {}
+    for k in ("wpudmax", "zwpmax", "bperm", "rhosoi_val", "csoi_mineral_val"):
+        if k in so:
+            setattr(sim, k, float(so[k]))
+    sim.h2o_consistent_weights = bool(so.get("h2o_consistent_weights", False))
+    fp = d.get("fire", {}) or {}
+    for k in ("d_crit", "vpd_ref", "d_crit_vpd"):
+        if k in fp:
+            setattr(sim.fire, k, float(fp[k]))
+    for tid, kv in (d.get("types", {}) or {}).items():
+        if tid not in TYPE_IDS:
+            raise KeyError(f"unknown plant type {tid!r}")
+        for key, attr in TYPE_KEYS.items():
+            if key in (kv or {}):
+                getattr(sim, attr)[TYPE_IDS.index(tid)] = float(kv[key])
+    return sim
+
+
+def load_params_yaml(path: str, bedrock: bool = True) -> SimParams:
+    """SimParams from one of the reference's parameter files (``configs/params/*.yaml``)."""
+    import yaml
+    with open(path) as fh:
+        return sim_params_from_dict(yaml.safe_load(fh) or {}, bedrock=bedrock)

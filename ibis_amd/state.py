@@ -17,7 +17,11 @@ import torch
 from . import layout as L
 from .params import SnowConstants
 
-_GROUPS = ("soil", "snow", "canopy", "atm", "vegetation", "hydrology", "diag")
+_GROUPS = ("soil", "snow", "canopy", "atm", "vegetation", "hydrology", "diag", "seasonal")
+#: optional per-cell leaves kept beside the arena (read by the optional StomataParams
+#: switches): the SeasonalState (python-port/ibis/state/seasonal.py:36-43) and the day's mean
+#: air temperature AtmState.td
+AUX_KEYS = ("seasonal.acclim_x", "seasonal.ssen_u", "seasonal.ssen_l", "atm.td")
 
 
 class _Group:
@@ -32,6 +36,8 @@ class _Group:
         if key in L.FIELD_INDEX:
             return self._state.field(key)
         st = self._state
+        if key in st.aux:
+            return st.aux[key][:st.n]
         if key == "soil.hsoi":
             return st.hsoi
         if key == "vegetation.froot":
@@ -46,6 +52,12 @@ class _Group:
 
     def __setattr__(self, name, value):
         key = f"{self._group}.{name}"
+        st = self._state
+        if key in AUX_KEYS:
+            if key not in st.aux:
+                st.aux[key] = torch.zeros(st.pitch, dtype=st.dtype, device=st.device)
+            st.aux[key][:st.n].copy_(torch.as_tensor(value, dtype=st.dtype, device=st.device).reshape(st.n))
+            return
         if key not in L.FIELD_INDEX:
             raise AttributeError(f"{key} is not a per-cell leaf of the hourly step's state")
         view = self._state.field(key)
@@ -55,7 +67,8 @@ class _Group:
 class IBISState:
     def __init__(self, arena: torch.Tensor, slot_type: torch.Tensor | None, n: int, hsoi, froot,
                  co2conc: float = 380.0e-6, nytimes: float = 0.0,
-                 snow_const: SnowConstants | None = None, bedrock: bool = True):
+                 snow_const: SnowConstants | None = None, bedrock: bool = True,
+                 aux: dict | None = None):
         if arena.dim() != 2 or arena.shape[0] != L.NPLANE or arena.shape[1] < n:
             raise ValueError(f"arena must be ({L.NPLANE}, pitch >= {n}), got {tuple(arena.shape)}")
         if slot_type is not None and (slot_type.shape[0] != L.NPFT or slot_type.shape[1] != arena.shape[1]
@@ -70,6 +83,11 @@ class IBISState:
         self.nytimes = float(nytimes)
         self.snow_const = snow_const or SnowConstants()
         self.bedrock = bool(bedrock)
+        self.aux = dict(aux or {})
+        for k, t in self.aux.items():
+            if k not in AUX_KEYS or t.shape != (arena.shape[1],) or t.dtype != arena.dtype:
+                raise ValueError(f"aux leaf {k!r} must be one of {AUX_KEYS}, shaped (pitch,) "
+                                 "in the arena's dtype")
         self._work = {}
 
     # -- sub-states -------------------------------------------------------------------
@@ -100,10 +118,12 @@ class IBISState:
     def like(self, arena: torch.Tensor) -> "IBISState":
         """A state sharing everything but the arena."""
         return IBISState(arena, self.slot_type, self.n, self.hsoi, self.froot, self.co2conc,
-                         self.nytimes, self.snow_const, self.bedrock)
+                         self.nytimes, self.snow_const, self.bedrock, self.aux)
 
     def clone(self) -> "IBISState":
-        return self.like(self.arena.clone())
+        new = self.like(self.arena.clone())
+        new.aux = {k: t.clone() for k, t in self.aux.items()}
+        return new
 
     # -- host conversion ----------------------------------------------------------------
     @classmethod
@@ -123,12 +143,18 @@ class IBISState:
                 else:
                     a[off:off + f.width, :n] = v.reshape(n, f.width).T
         arena = torch.from_numpy(a).to(device)
+        aux = {}
+        for k in AUX_KEYS:
+            if k in fields:
+                v = np.zeros(pitch, dtype=np_dt)
+                v[:n] = np.asarray(fields[k], dtype=np.float64).reshape(n)
+                aux[k] = torch.from_numpy(v).to(device)
         st = None
         if slot_type is not None:
             s = np.full((L.NPFT, pitch), -1, dtype=np.int8)
             s[:, :n] = np.asarray(slot_type, dtype=np.int8).T
             st = torch.from_numpy(s).to(device)
-        return cls(arena, st, n, hsoi, froot, co2conc, nytimes, snow_const, bedrock)
+        return cls(arena, st, n, hsoi, froot, co2conc, nytimes, snow_const, bedrock, aux)
 
     def to_numpy(self, keys=None) -> dict:
         a = self.arena[:, :self.n].detach().cpu().numpy().astype(np.float64)

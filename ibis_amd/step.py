@@ -3,8 +3,9 @@
 Drop-in for ``python-port/ibis/timestepping/loops.py:234-1284``: forcing is pre-set in
 ``state.atm`` (``loops.py:242-243``), the step returns a new state and leaves its input
 untouched (``loops.py:1278-1284``).  ``timestep_`` is the in-place form for drivers that
-own their buffers.  Both run the HIP kernel through the C ABI (``ibis_step`` of
-``include/ibis_step.h``) on the current torch stream of the state's device; there is no
+own their buffers.  Both run the HIP kernels through the C ABI (``ibis_step`` of
+``include/ibis_step.h``; ``ibis_step_ext`` when an optional StomataParams switch is on or
+``stressfac`` is per cell) on the current torch stream of the state's device; there is no
 host fallback.
 """
 from __future__ import annotations
@@ -43,6 +44,75 @@ def type_table(params: SimParams, device) -> torch.Tensor:
     return t
 
 
+#: per-slot tables of ibis_step_ext: (StomataParams field, SimParams per-type overrides)
+SLOT_TABLES = (("vmax_pft", "vmax_type_overrides"), ("coefm_pft", "coefm_type_overrides"),
+               ("acclim_s0_pft", "acclim_s0_type_overrides"),
+               ("acclim_smax_pft", "acclim_smax_type_overrides"))
+
+
+def slot_tables(params: SimParams) -> np.ndarray:
+    """The IBIS_NTABLE x 52 per-slot tables of ibis_step_ext on the host: for each of
+    vmax_pft, coefm_pft, acclim_s0_pft and acclim_smax_pft, the 40 zone-type rows (the
+    override, else the parent position's value) and the 12 position rows
+    (python-port/ibis/io/pft_types.py:170-189)."""
+    sp = params.stomata_params
+    parts = []
+    for name, ovr in SLOT_TABLES:
+        base = np.asarray(getattr(sp, name), dtype=np.float64)
+        if base.shape != (L.NPFT,):
+            raise ValueError(f"stomata_params.{name} must be the (12,) base vector; per-cell "
+                             "values come from the state's slot types")
+        parts.append(synth.type_table(base, dict(getattr(params, ovr))))
+    return np.concatenate(parts)
+
+
+def type_tables(params: SimParams, device) -> torch.Tensor:
+    """Device copy of ``slot_tables`` (cached per device and values)."""
+    host = slot_tables(params)
+    key = (str(device), "ext", host.tobytes())
+    t = _table_cache.get(key)
+    if t is None:
+        t = torch.from_numpy(host).to(device)
+        _table_cache[key] = t
+    return t
+
+
+def _cell_plane(state: IBISState, values, what: str) -> torch.Tensor:
+    """A per-cell host array as a (pitch,) device plane of the arena's dtype (cached)."""
+    v = np.asarray(values, dtype=np.float64)
+    if v.shape != (state.n,):
+        raise ValueError(f"{what} must be a scalar or ({state.n},), got {v.shape}")
+    key = ("cell", what, str(state.device), state.dtype, state.pitch, v.tobytes())
+    t = _table_cache.get(key)
+    if t is None:
+        t = torch.zeros(state.pitch, dtype=state.dtype)
+        t[:state.n] = torch.from_numpy(v)
+        t = t.to(state.device)
+        if len(_table_cache) > 64:
+            _table_cache.clear()
+        _table_cache[key] = t
+    return t
+
+
+def ext_inputs(state: IBISState, params: SimParams):
+    """(switch block, tables, acclim_x, td, stressfac) of ibis_step_ext for ``state``.
+
+    The acclimation factor needs the seasonal state (timestepping/loops.py:294: without
+    ``state.seasonal`` the F77 Vmax is kept); the frost ramp needs ``atm.td``."""
+    sp = params.stomata_params
+    x = abi.stomata_ext(sp)
+    acclim_x = state.aux.get("seasonal.acclim_x")
+    if acclim_x is None:
+        x.switches &= ~abi.SW_ACCLIM
+    td = state.aux.get("atm.td") if (x.switches & abi.SW_ACCLIM and sp.frost_on) else None
+    if x.switches & abi.SW_ACCLIM and sp.frost_on and td is None:
+        raise ValueError("frost_on needs the daily mean air temperature state.atm.td")
+    sf = None
+    if np.ndim(sp.stressfac) != 0:
+        sf = _cell_plane(state, sp.stressfac, "stomata_params.stressfac")
+    return x, type_tables(params, state.device), acclim_x, td, sf
+
+
 def ibis_params(state: IBISState, consts: PhysicalConstants, params: SimParams,
                 flux_max: float = 2000.0, t_max_dev: float = 40.0) -> abi.IbisParams:
     return abi.params_from(consts, params, state.snow_const, state.hsoi, state.froot,
@@ -74,23 +144,26 @@ def workspace(state: IBISState, code: int, stream) -> torch.Tensor:
     return ws
 
 
+def _check(state_in: IBISState, params):
+    if state_in.device.type != "cuda":
+        raise RuntimeError("the hourly step runs on the GPU; move the state with "
+                           "IBISState.from_numpy(..., device='cuda')")
+    if state_in.bedrock and params.bedrock is None:
+        raise ValueError("state carries bedrock temperatures but params.bedrock is None "
+                         "(python-port/ibis/physics/bedrock.py:120-124)")
+
+
 def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: float,
            precision=None, copy_in=False, stream=None, cparams=None):
     """Enqueue one hour on ``stream`` (default: torch's current stream of the device)."""
     lib = _lib.load()
-    if state_in.device.type != "cuda":
-        raise RuntimeError("the hourly step runs on the GPU; move the state with "
-                           "IBISState.from_numpy(..., device='cuda')")
+    _check(state_in, params)
     code = _precision(state_in, precision)
     if arena_out.shape != state_in.arena.shape or arena_out.dtype != state_in.dtype:
         raise ValueError("output arena must match the input arena's shape and dtype")
     if not state_in.arena.is_contiguous() or not arena_out.is_contiguous():
         raise ValueError("arenas must be contiguous")
-    if state_in.bedrock and params.bedrock is None:
-        raise ValueError("state carries bedrock temperatures but params.bedrock is None "
-                         "(python-port/ibis/physics/bedrock.py:120-124)")
     flags = code | (abi.FLAG_BEDROCK if state_in.bedrock else 0) | (abi.FLAG_COPY_IN if copy_in else 0)
-    table = type_table(params, state_in.device)
     p = cparams if cparams is not None else ibis_params(state_in, consts, params)
     ain = abi.IbisArena(state_in.arena.data_ptr(), state_in.pitch)
     aout = abi.IbisArena(arena_out.data_ptr(), state_in.pitch)
@@ -98,9 +171,22 @@ def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: 
         stream = torch.cuda.current_stream(state_in.device)
     slot = state_in.slot_type
     ws = workspace(state_in, code, stream)
+    slot_ptr = C.c_void_p(slot.data_ptr() if slot is not None else 0)
+    if params.stomata_params.switches_on():
+        x, tables, acclim_x, td, sf = ext_inputs(state_in, params)
+        ptr = lambda t: t.data_ptr() if t is not None else 0
+        x.type_tables, x.acclim_x, x.td, x.stressfac = (tables.data_ptr(), ptr(acclim_x),
+                                                          ptr(td), ptr(sf))
+        rc = lib.ibis_step_ext(C.byref(ain), C.byref(aout), C.c_void_p(ws.data_ptr()),
+                               C.c_int64(ws.numel()), slot_ptr, C.c_int64(state_in.pitch),
+                               C.byref(p), C.byref(x), C.c_double(dtime),
+                               C.c_double(state_in.nytimes), C.c_int64(state_in.n),
+                               C.c_int(flags), C.c_void_p(stream.cuda_stream))
+        _lib.check(rc, "ibis_step_ext")
+        return
+    table = type_table(params, state_in.device)
     rc = lib.ibis_step(C.byref(ain), C.byref(aout), C.c_void_p(ws.data_ptr()),
-                       C.c_int64(ws.numel()),
-                       C.c_void_p(slot.data_ptr() if slot is not None else 0),
+                       C.c_int64(ws.numel()), slot_ptr,
                        C.c_int64(state_in.pitch), C.c_void_p(table.data_ptr()), C.byref(p),
                        C.c_double(dtime), C.c_double(state_in.nytimes),
                        C.c_int64(state_in.n), C.c_int(flags), C.c_void_p(stream.cuda_stream))
@@ -109,9 +195,24 @@ def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: 
 
 def timestep_body(state: IBISState, consts: PhysicalConstants, params: SimParams, dtime: float,
                   precision=None) -> IBISState:
-    """One hour, functional: returns a new state, ``state`` is not modified."""
-    out = torch.empty_like(state.arena)
-    launch(state, out, consts, params, dtime, precision=precision, copy_in=True)
+    """One hour, functional: returns a new state, ``state`` is not modified.  Runs the
+    registered operator ``torch.ops.ibis.step`` (ibis_amd.ops)."""
+    from . import ops  # registers torch.ops.ibis.*
+    _check(state, params)
+    code = _precision(state, precision)
+    flags = code | (abi.FLAG_BEDROCK if state.bedrock else 0)
+    ws = workspace(state, code, torch.cuda.current_stream(state.device))
+    p = ibis_params(state, consts, params)
+    if params.stomata_params.switches_on():
+        x, tables, acclim_x, td, sf = ext_inputs(state, params)
+        out = torch.ops.ibis.step_ext(state.arena, ws, state.slot_type, tables,
+                                      abi.params_bytes(p), ops.ext_block(x), acclim_x, td, sf,
+                                      float(dtime), float(state.nytimes), int(state.n),
+                                      int(flags))
+    else:
+        out = torch.ops.ibis.step(state.arena, ws, state.slot_type,
+                                  type_table(params, state.device), abi.params_bytes(p),
+                                  float(dtime), float(state.nytimes), int(state.n), int(flags))
     new = state.like(out)
     new.nytimes = state.nytimes + 1.0
     return new

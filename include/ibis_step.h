@@ -36,7 +36,8 @@ typedef struct IbisArena {
 
 /* Scalar parameters, by value.  Field names are the reference's:
  *   PhysicalConstants  python-port/ibis/constants.py:16-79
- *   StomataParams      python-port/ibis/physics/stomata.py:33-161 (F77 switches)
+ *   StomataParams      python-port/ibis/physics/stomata.py:33-161 (the switches of
+ *                      stomata.py:93-161 travel in IbisStomataExt)
  *   CanopyParams       python-port/ibis/physics/canopy_exchange.py:68-84
  *   InterceptionParams python-port/ibis/physics/interception.py:31-45
  *   VegRadParams       python-port/ibis/physics/radiation.py:32-56
@@ -95,7 +96,7 @@ typedef struct IbisParams {
 
 /* Bytes of device workspace ibis_step needs for arenas of `pitch` cells at the precision
  * selected by `flags` (the values handed between the step's phases that are not state
- * fields: 34 planes in the solve precision, 25 in the bulk precision).  Negative on bad
+ * fields: 34 planes in the solve precision, 27 in the bulk precision).  Negative on bad
  * arguments.  No counterpart in the reference (its JAX step keeps these in XLA buffers). */
 int64_t ibis_step_workspace_bytes(int64_t pitch, int flags);
 
@@ -114,6 +115,53 @@ int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
               int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
               const double* type_table, const IbisParams* params, double dtime, double nytimes,
               int64_t ncell, int flags, void* hip_stream);
+
+/* Optional StomataParams switches (python-port/ibis/physics/stomata.py:93-161, static
+ * fields chosen at trace time in the reference; here bits of IbisStomataExt.switches).  With
+ * none set, ibis_step_ext performs the arithmetic of ibis_step. */
+#define IBIS_SW_VMAX_MIX 0x001        /* vmax_mix: composition-weighted Vmax per big leaf */
+#define IBIS_SW_COEFM_MIX 0x002       /* coefm_mix: composition-weighted Ball-Berry slope */
+#define IBIS_SW_SUNSHADE_U 0x004      /* sunshade_u: sunlit/shaded upper canopy */
+#define IBIS_SW_SUNSHADE_L 0x008      /* sunshade_l */
+#define IBIS_SW_CAP_SCALE_U 0x010     /* cap_scale_u: capacity with its own profile */
+#define IBIS_SW_CAP_SCALE_L 0x020     /* cap_scale_l */
+#define IBIS_SW_SCALE_LEAF_ONLY 0x040 /* scale_leaf_only: k (L + S) scaling denominator */
+#define IBIS_SW_ACCLIM 0x080          /* acclim_on with a seasonal state (loops.py:291-310) */
+#define IBIS_SW_ACCLIM_PER_PFT 0x100  /* acclim_per_pft: s0 / smax per plant type */
+#define IBIS_SW_ACCLIM_LIGHT 0x200    /* acclim_light: light-limited rate scaled too */
+#define IBIS_SW_FROST 0x400           /* frost_on: frost ramp on the day's mean temperature */
+#define IBIS_SW_ALL 0x7ff
+
+/* Per-slot tables of ibis_step_ext, each IBIS_TYPE_ROWS doubles with the row rule of the
+ * Vmax table: vmax_pft, coefm_pft, acclim_s0_pft, acclim_smax_pft (the per-cell (npoi, 12)
+ * tables of python-port/ibis/io/pft_types.py:205-212). */
+#define IBIS_NTABLE 4
+
+typedef struct IbisStomataExt {
+  int32_t switches;            /* IBIS_SW_* */
+  int32_t acclim_mask;         /* bit j: plant type j acclimates (StomataParams.acclim_pfts) */
+  double cap_kn;               /* capacity extinction Kn (sunshade, cap_scale) */
+  double acclim_s0, acclim_smax; /* acclimation constants without IBIS_SW_ACCLIM_PER_PFT */
+  double frost_t0, frost_t1;   /* frost ramp, deg C */
+  const double* type_tables;   /* IBIS_NTABLE x IBIS_TYPE_ROWS doubles, device memory */
+  const void* acclim_x;        /* SeasonalState.acclim_x (deg C), ncell elements of the
+                                  arena's type; read with IBIS_SW_ACCLIM */
+  const void* td;              /* AtmState.td (K), ncell elements; read with IBIS_SW_FROST */
+  const void* stressfac;       /* per-cell stressfac, ncell elements, or NULL for
+                                  IbisParams.stressfac (the per-zone scalar of
+                                  python-port/ibis/io/pft_types.py:218-230) */
+} IbisStomataExt;
+
+/* ibis_step with the optional StomataParams switches and per-slot tables of `ext` (device
+ * pointers inside it; `ext` itself is host memory).  Same arena, workspace, flags and error
+ * codes as ibis_step; the sunlit/shaded canopies read the two-stream's unscattered beam term
+ * (radiation.py:516, :527), which the surface kernel hands over in the workspace.
+ * Replaces timestep_body with StomataParams switches on and per-cell type tables
+ * (python-port/ibis/timestepping/loops.py:291-310, :499-516; physics/stomata.py:732-1080). */
+int ibis_step_ext(const IbisArena* in, const IbisArena* out, void* workspace,
+                  int64_t workspace_bytes, const int8_t* slot_type, int64_t slot_pitch,
+                  const IbisParams* params, const IbisStomataExt* ext, double dtime,
+                  double nytimes, int64_t ncell, int flags, void* hip_stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* ibis_last_error(void);

@@ -1,8 +1,10 @@
 """Convenience driver of the oracle step -- TEST INFRASTRUCTURE.
 
 ``run_step`` applies the per-cell plant-type gather the reference does on the host
-(``python-port/ibis/io/pft_types.py:170-189``: a slot holding zone type t takes the Vmax of
-t's parent PFT unless overridden; a negative id takes the position's own value) and calls
+(``python-port/ibis/io/pft_types.py:170-189``: a slot holding zone type t takes the value of
+t's parent PFT unless overridden; a negative id takes the position's own value) to Vmax and,
+when the parameter set has them, the per-type stomatal slopes and acclimation constants, and
+calls
 ``oracle.step.timestep_body`` with reference-named parameter objects.
 """
 from __future__ import annotations
@@ -21,9 +23,10 @@ TYPE_PARENT = np.array([0, 1, 1, 2, 1, 2, 3, 4, 2, 3, 4, 4, 3, 2, 4, 3, 5, 6, 7,
                         10, 11, 11, 11, 10])
 
 
-def gather_vmax(slot_type, vmax_base, overrides=None):
-    """(N, 12) per-cell Vmax from slot type ids (pft_types.gather)."""
-    base = np.asarray(vmax_base, dtype=np.float64)
+def gather(slot_type, base, overrides=None):
+    """(N, 12) per-cell parameter from slot type ids (pft_types.gather): a slot holding type t
+    takes overrides[t], else the value of t's parent position; a negative id its own."""
+    base = np.asarray(base, dtype=np.float64)
     val = base[TYPE_PARENT].copy()
     for t, v in (overrides or {}).items():
         val[int(t)] = float(v)
@@ -32,14 +35,28 @@ def gather_vmax(slot_type, vmax_base, overrides=None):
     return np.where(s >= 0, val[np.clip(s, 0, 39)], own)
 
 
+def gather_vmax(slot_type, vmax_base, overrides=None):
+    """(N, 12) per-cell Vmax from slot type ids (pft_types.gather)."""
+    return gather(slot_type, vmax_base, overrides)
+
+
+#: per-slot StomataParams tables gathered like Vmax (io/pft_types.py:205-212):
+#: (field, overrides attribute of the parameter set)
+SLOT_TABLES = (("vmax_pft", "vmax_type_overrides"), ("coefm_pft", "coefm_type_overrides"),
+               ("acclim_s0_pft", "acclim_s0_type_overrides"),
+               ("acclim_smax_pft", "acclim_smax_type_overrides"))
+
+
 def run_step(fields, slot_type, consts=None, params=None, dtime=3600.0, hsoi=None, froot=None,
              co2conc=380.0e-6, nytimes=0.0, snow=None, bedrock=True, knife_sides=None):
     consts = consts or OP.physical_constants()
     params = params or OP.sim_params()
     p = copy.copy(params)
     sp = copy.copy(params.stomata_params)
-    sp.vmax_pft = gather_vmax(slot_type, params.stomata_params.vmax_pft,
-                              getattr(params, "vmax_type_overrides", None))
+    for name, ovr in SLOT_TABLES:
+        base = getattr(params.stomata_params, name, None)
+        if base is not None:
+            setattr(sp, name, gather(slot_type, base, getattr(params, ovr, None)))
     p.stomata_params = sp
     if not bedrock:
         p.bedrock = None

@@ -4,8 +4,10 @@
 arrays ``{"soil.tsoi": (N, 6), "snow.fi": (N,), ...}`` (the fields of
 ``ibis_amd.layout.FIELDS`` with role ``in``/``io``) to a dict of every ``io``/``out`` field.
 ``glob`` carries the leaves that are not per cell: ``hsoi`` (7,), ``froot`` (6, 2),
-``co2conc``, ``nytimes`` and the snow constants.  The F77 path is restated (all optional
-StomataParams switches off; ``h2o_consistent_weights`` honoured).
+``co2conc``, ``nytimes`` and the snow constants.  The F77 path and the optional
+StomataParams switches are restated (``oracle.stomata``); ``h2o_consistent_weights`` is
+honoured.  With acclimation on, ``st`` may carry ``seasonal.acclim_x`` (and ``atm.td`` for
+the frost ramp); a per-cell ``stressfac`` (N,) is taken as given.
 
 Pinned, as a whole, by: column water balance closure (``tests/test_water_balance.py:35-47``);
 no negative puddle on a dry surface (``tests/test_puddle_conservation.py:95-186``); night
@@ -62,6 +64,8 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
     fira, raina, snowa = g("atm.fira"), g("atm.raina"), g("atm.snowa")
     lai, sai, fu, fl = g("vegetation.lai"), g("vegetation.sai"), g("vegetation.fu"), g("vegetation.fl")
     frac, exist = g("vegetation.frac"), g("vegetation.exist")
+    # acclimation factor of Vmax from the optional seasonal state (loops.py:291-310)
+    vfac = stom_.acclim_factor(st.get("seasonal.acclim_x"), st.get("atm.td"), sp, frac.shape[0])
     stores = [g("canopy." + k) for k in ("wliqu", "wliqs", "wliql", "wsnou", "wsnos", "wsnol")]
 
     # 1. diagnostic soil properties (loops.py:318-326)
@@ -131,7 +135,8 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
         tc = can_.turcof(a_it, ta, qa, ua, t12, t34, q12, q34, c, cp_, dtime, first, carry)
         stom = stom_.stomata(tu, tl, t12, t34, q12, q34, tc.su, tc.sl, psurf, rad, ci_u, gs_u,
                              cs_u, ci_l, gs_l, cs_l, fw2.fwetu, fw2.fwetl, lai, sai, exist,
-                             stresstu, stresstl, a10, glob.co2conc, dtime, sp, eps)
+                             stresstu, stresstl, a10, glob.co2conc, dtime, sp, eps,
+                             frac=frac, vfac=vfac)
         ci_u, gs_u, cs_u = stom.ci_upper, stom.gs_upper, stom.cs_upper
         ci_l, gs_l, cs_l = stom.ci_lower, stom.gs_lower, stom.cs_lower
         a10 = stom.a10

@@ -1,12 +1,22 @@
 """Photosynthesis, stomatal conductance and soil-water stress (FP64), restating
-``python-port/ibis/physics/stomata.py`` on its F77 path (vmax_mix, coefm_mix, cap_scale,
-sunshade, scale_leaf_only and acclimation off).
+``python-port/ibis/physics/stomata.py``: the F77 path and the optional ``StomataParams``
+switches (``stomata.py:93-161``): composition-weighted Vmax (``vmax_mix``) and stomatal slope
+(``coefm_mix``), the leaf-only scaling denominator (``scale_leaf_only``), capacity scaling
+(``cap_scale_u/l``), the sunlit/shaded two-big-leaf canopy (``sunshade_u/l``), the
+acclimation factor of Vmax (``vfac``) and its light-rate factor (``acclim_light``).  A switch
+is read with ``getattr(sp, name, False)``: a parameter object without it is the F77 path.
 
 Pinned by: ``colimit`` against the smaller root of theta x^2 - (j1+j2) x + j1 j2 from the
 quadratic formula (textbook; ``stomata.py:198-221``); drystress bounds and root-weighted sum
-(``tests/test_stomata.py``); ``_total_conductance`` as a series resistance.  The composed
-``stomata`` outputs have only bound/sign checks in the reference
-(``tests/test_stomata.py:404-535``): parity unpinned beyond those.
+(``tests/test_stomata.py``); ``_total_conductance`` as a series resistance.  The switches by
+the reference's own tests (``tests/test_sunshade_canopy.py``, ``test_canopy_capacity.py``,
+``test_vmax_mix.py``, ``test_pft_slopes_acclim.py``, ``test_cold_season.py``): the sunlit and
+shaded partition conserves light, leaf area and capacity and integrates the two-stream
+profile to the absorbed flux; uniform light and capacity reduce the two big leaves to one;
+equal light and capacity profiles reproduce the F77 rate times the profile; one-hot
+compositions give each plant type's own Vmax and slope; acclimation and light factors of
+one leave the rates unchanged.  The composed ``stomata`` outputs have only bound/sign checks
+in the reference (``tests/test_stomata.py:404-535``): parity unpinned beyond those.
 """
 from __future__ import annotations
 
@@ -26,20 +36,31 @@ def colimit(j1, j2, theta):
     return np.minimum(q / theta, cc / q)
 
 
-def farquhar_c3(par, ci, vmax, gamstar, kc, ko, o2, rdark, sp):
-    """C3 leaf rates (stomata.py:228-272)."""
+def farquhar_c3(par, ci, vmax, gamstar, kc, ko, o2, rdark, sp, f_light=None, f_cap=None,
+                je_fac=None):
+    """C3 leaf rates (stomata.py:228-272).  ``je_fac`` scales the light-limited rate
+    (acclim_light); with ``f_cap`` the light limit is scaled by ``f_light`` and the Rubisco and
+    sink limits and dark respiration by ``f_cap`` before co-limitation (cap_scale)."""
     je = par * 4.59e-06 * sp.alpha3 * (ci - gamstar) / (ci + 2.0 * gamstar)
+    if je_fac is not None:
+        je = je * je_fac
     jc = vmax * (ci - gamstar) / (ci + kc * (1.0 + o2 / ko))
     js = vmax / 2.2
+    if f_cap is not None:
+        je, jc, js, rdark = je * f_light, jc * f_cap, js * f_cap, rdark * f_cap
     ag = colimit(colimit(je, jc, sp.theta3), js, sp.beta3)
     return ag, ag - rdark
 
 
-def farquhar_c4(par, ci, vmax, rdark, sp):
-    """C4 leaf rates (stomata.py:279-314)."""
+def farquhar_c4(par, ci, vmax, rdark, sp, f_light=None, f_cap=None, je_fac=None):
+    """C4 leaf rates (stomata.py:279-314), with the factors of ``farquhar_c3``."""
     je = par * 4.59e-06 * sp.alpha4
+    if je_fac is not None:
+        je = je * je_fac
     jc = vmax
     ji = 18.0e3 * vmax * ci
+    if f_cap is not None:
+        je, jc, ji, rdark = je * f_light, jc * f_cap, ji * f_cap, rdark * f_cap
     ag = colimit(colimit(je, jc, sp.theta4), ji, sp.beta4)
     return ag, ag - rdark
 
@@ -56,8 +77,9 @@ def ball_berry(an, cs0, gs0, ci0, co2, gbco2, rh, coefm, coefb, stress, gsmin, g
     return cs, gs, ci
 
 
-def canopy_scaling(toppar, t6, t7, sc, lai_v, sai_v, a10s, a10d, dtime, eps):
-    """PAR-extinction canopy scaling and 10-day means (stomata.py:442-508), F77 denominator."""
+def canopy_scaling(toppar, t6, t7, sc, lai_v, sai_v, a10s, a10d, dtime, eps, leaf_only=False):
+    """PAR-extinction canopy scaling and 10-day means (stomata.py:442-508).  F77 divides the
+    absorbed fraction by k L; ``leaf_only`` (scale_leaf_only) by k (L + S)."""
     ext = (t6 * sc[:, 0] + t7 * sc[:, 1] - t7 * sc[:, 2]) / np.maximum(sc[:, 3], eps)
     ext = np.clip(ext, 0.1, 10.0)
     pxai = ext * (lai_v + sai_v)
@@ -65,13 +87,155 @@ def canopy_scaling(toppar, t6, t7, sc, lai_v, sai_v, a10s, a10d, dtime, eps):
     zw = np.exp(-1.0 / (10.0 * 86400.0 / dtime))
     has = plai > 0.0
     day = toppar > 10.0
-    sday = (1.0 - np.exp(-pxai)) / np.maximum(plai, eps)
+    sday = (1.0 - np.exp(-pxai)) / np.maximum(pxai if leaf_only else plai, eps)
     upd = has & day
     a10s_n = np.where(upd, zw * a10s + (1.0 - zw) * sday * toppar, a10s)
     a10d_n = np.where(upd, zw * a10d + (1.0 - zw) * toppar, a10d)
     snight = a10s / np.maximum(a10d, eps)
     scale = np.where(has, np.where(day, sday, snight), 0.0)
     return scale, a10s_n, a10d_n
+
+
+# plant-type positions of the three mixed big leaves (stomata.py:569-571)
+BROADLEAF = (0, 1, 2, 4, 6)
+CONIFER = (3, 5, 7)
+SHRUB = (8, 9)
+
+
+def capacity_integral(lai_v, kn):
+    """Canopy-mean relative capacity (1 - e^-Kn L) / (Kn L) of an exponential capacity profile;
+    1 - x/2 below x = 1e-4, 0 without leaves (stomata.py:379-384)."""
+    x = kn * lai_v
+    f = np.where(x > 1.0e-4, -np.expm1(-x) / np.maximum(x, 1.0e-4), 1.0 - 0.5 * x)
+    return np.where(lai_v > 0.0, f, 0.0)
+
+
+def e1(a, plai):
+    """Integral of e^-a x over [0, plai]: (1 - e^-a plai) / a, and plai (1 - a plai / 2) where
+    |a plai| < 1e-5 (stomata.py:387-392)."""
+    x = a * plai
+    small = np.abs(x) < 1.0e-5
+    return np.where(small, plai * (1.0 - 0.5 * x), -np.expm1(-x) / np.where(small, 1.0, a))
+
+
+def sunshade_canopy(sc, dbeam, k, h, lai_v, sai_v, kn, eps):
+    """Sunlit / shaded partition of one canopy (stomata.py:395-424): absorbed PAR of the sunlit
+    and shaded leaves per unit ground, their leaf areas and their capacities in top-leaf units.
+    The two-stream's absorbed profile per unit plant area at depth x is A e^-kx + B e^-hx +
+    C e^hx; sunlit area at depth x is e^-kx and takes the unscattered beam ``dbeam`` as well."""
+    A, B, C = sc[:, 0], sc[:, 1], sc[:, 2]
+    plai = np.maximum(lai_v + sai_v, eps)
+    fl = np.where(lai_v > 0.0, lai_v / plai, 0.0)
+    ek, e2k = e1(k, plai), e1(2.0 * k, plai)
+    tot = np.maximum(A * ek + B * e1(h, plai) + C * e1(-h, plai), 0.0)
+    sun = A * e2k + B * e1(k + h, plai) + C * e1(k - h, plai) + dbeam * (ek - e2k)
+    sun = np.clip(sun, 0.0, tot)
+    sha = tot - sun
+    lai_sun = fl * ek
+    lai_sha = np.maximum(lai_v - lai_sun, 0.0)
+    v_sun = fl * e1(kn * fl + k, plai)
+    v_sha = np.maximum(fl * e1(kn * fl, plai) - v_sun, 0.0)
+    return sun * fl, sha * fl, lai_sun, lai_sha, v_sun, v_sha
+
+
+def _per_leaf_area(lai_v):
+    return np.where(lai_v > 0.0, 1.0 / np.maximum(lai_v, 1.0e-6), 0.0)
+
+
+def sunshade_c3(ss, ci, vmax, gamstar, kc, ko, o2, rdark, lai_v, sp, je_fac=None):
+    """Canopy-mean C3 rates per unit leaf area from the two big leaves, each one leaf-model call
+    on its totals (stomata.py:427-434)."""
+    a_sun, a_sha, _, _, v_sun, v_sha = ss
+    g1, n1 = farquhar_c3(a_sun, ci, vmax * v_sun, gamstar, kc, ko, o2, rdark * v_sun, sp,
+                         je_fac=je_fac)
+    g2, n2 = farquhar_c3(a_sha, ci, vmax * v_sha, gamstar, kc, ko, o2, rdark * v_sha, sp,
+                         je_fac=je_fac)
+    inv = _per_leaf_area(lai_v)
+    return (g1 + g2) * inv, (n1 + n2) * inv
+
+
+def sunshade_c4(ss, ci, vmax, rdark, lai_v, sp, je_fac=None):
+    """As ``sunshade_c3`` for the C4 leaf model (stomata.py:437-443)."""
+    a_sun, a_sha, _, _, v_sun, v_sha = ss
+    g1, n1 = farquhar_c4(a_sun, ci, vmax * v_sun, rdark * v_sun, sp, je_fac=je_fac)
+    g2, n2 = farquhar_c4(a_sha, ci, vmax * v_sha, rdark * v_sha, sp, je_fac=je_fac)
+    inv = _per_leaf_area(lai_v)
+    return (g1 + g2) * inv, (n1 + n2) * inv
+
+
+def mix_pft(frac, vp, idx, fallback, eps=1.0e-6):
+    """Displayed-LAI-weighted mean of the per-cell (N, 12) ``vp`` over positions ``idx``;
+    ``fallback`` where they hold no leaves (stomata.py:587-595)."""
+    ix = list(idx)
+    w = frac[:, ix]
+    tot = w[:, 0]
+    num = w[:, 0] * vp[:, ix[0]]
+    for j in range(1, len(ix)):
+        tot = tot + w[:, j]
+        num = num + w[:, j] * vp[:, ix[j]]
+    return np.where(tot > eps, num / np.maximum(tot, eps), fallback)
+
+
+def capacity_shares(frac, vp, idx, eps=1.0e-12):
+    """(N, len(idx)) shares of a big leaf's GPP by displayed leaf area x own Vmax, rescaled to
+    the leaf-area total over ``idx``; plain leaf-area shares where no capacity
+    (stomata.py:574-584)."""
+    ix = list(idx)
+    f = frac[:, ix]
+    w = f * vp[:, ix]
+    tw = w.sum(axis=1, keepdims=True)
+    tf = f.sum(axis=1, keepdims=True)
+    return np.where(tw > eps, w / np.maximum(tw, eps) * tf, f)
+
+
+def select_vmax(exist, vp, frac=None, mix=False):
+    """Nominal Vmax of the five big leaves (ub, uc, ls, l3, l4) from per-cell (N, 12) ``vp``:
+    existence priority for the upper leaves and PFT 8 for the shrub (F77), or the
+    composition mean with the priority value as fallback (stomata.py:598-650)."""
+    ub = np.where(exist[:, 0] > 0.5, vp[:, 0], np.where(exist[:, 2] > 0.5, vp[:, 2], vp[:, 4]))
+    uc = np.where(exist[:, 3] > 0.5, vp[:, 3], vp[:, 5])
+    ls = vp[:, 8]
+    if mix and frac is not None:
+        ub = mix_pft(frac, vp, BROADLEAF, ub)
+        uc = mix_pft(frac, vp, CONIFER, uc)
+        ls = mix_pft(frac, vp, SHRUB, ls)
+    return ub, uc, ls, vp[:, 11], vp[:, 10]
+
+
+def slopes(sp, frac, n):
+    """Ball-Berry slopes of the five big leaves (ub, uc, ls, l3, l4): the group values, or with
+    coefm_mix the composition mean of the per-cell ``coefm_pft`` (stomata.py:656-664)."""
+    cm = np.asarray(sp.coefm, dtype=np.float64)
+    if not getattr(sp, "coefm_mix", False) or frac is None:
+        return tuple(np.full(n, v) for v in (cm[0], cm[1], cm[2], cm[4], cm[3]))
+    cp = np.broadcast_to(np.asarray(sp.coefm_pft, dtype=np.float64), (n, 12))
+    return (mix_pft(frac, cp, BROADLEAF, cm[0]), mix_pft(frac, cp, CONIFER, cm[1]),
+            mix_pft(frac, cp, SHRUB, cm[2]), cp[:, 11].copy(), cp[:, 10].copy())
+
+
+def acclim_factor(acclim_x, td, sp, n):
+    """(N, 12) acclimation factor of Vmax (timestepping/loops.py:291-310): f_S =
+    clip((S - s0) / smax, 0, 1) on the plant types of ``acclim_pfts`` (constants per type with
+    acclim_per_pft), times the frost ramp clip((td - 273.16 - t0) / (t1 - t0), 0, 1) on the same
+    types with frost_on; 1 elsewhere.  None when acclimation is off or there is no seasonal
+    state."""
+    if acclim_x is None or not getattr(sp, "acclim_on", False):
+        return None
+    mask = np.zeros(12)
+    mask[list(sp.acclim_pfts)] = 1.0
+    x = np.asarray(acclim_x, dtype=np.float64)
+    if getattr(sp, "acclim_per_pft", False):
+        s0 = np.broadcast_to(np.asarray(sp.acclim_s0_pft, dtype=np.float64), (n, 12))
+        sm = np.broadcast_to(np.asarray(sp.acclim_smax_pft, dtype=np.float64), (n, 12))
+        fs = np.clip((x[:, None] - s0) / sm, 0.0, 1.0)
+    else:
+        fs = np.clip((x - sp.acclim_s0) / sp.acclim_smax, 0.0, 1.0)[:, None]
+    v = 1.0 - mask[None, :] + mask[None, :] * fs
+    if getattr(sp, "frost_on", False):
+        ff = np.clip((np.asarray(td, dtype=np.float64) - 273.16 - sp.frost_t0)
+                     / (sp.frost_t1 - sp.frost_t0), 0.0, 1.0)
+        v = v * (1.0 - mask[None, :] + mask[None, :] * ff[:, None])
+    return v
 
 
 def drystress(wsoi, wisoi, swilt, sfield, froot, stressfac):
@@ -121,97 +285,111 @@ def vmax_table(sp, n):
 
 
 def stomata(tu, tl, t12, t34, q12, q34, su, sl, psurf, rad, ci_u, gs_u, cs_u, ci_l, gs_l,
-            cs_l, fwetu, fwetl, lai, sai, exist, stresstu, stresstl, a10, co2, dtime, sp, eps):
-    """Five big leaves (stomata.py:732-1080), F77 switches.  ``a10`` = (spu, dlu, spl, dll)."""
+            cs_l, fwetu, fwetl, lai, sai, exist, stresstu, stresstl, a10, co2, dtime, sp, eps,
+            frac=None, vfac=None):
+    """Five big leaves (stomata.py:732-1080).  ``a10`` = (spu, dlu, spl, dll); ``frac`` the
+    displayed-LAI shares (vegetation.frac), ``vfac`` the (N, 12) acclimation factor or None;
+    the sunlit/shaded canopies read ``rad.dbeamu`` / ``rad.dbeaml``."""
     n = tu.shape[0]
-    vp = vmax_table(sp, n)
-    gam, cm, cb, gsmin = sp.gamma, sp.coefm, sp.coefb, sp.gsmin
-
-    kc, ko, gstar, tv3, _ = _temp_dep(tu, sp)
-    gmin = 1.05 * gstar
-    ci_ub = np.clip(ci_u[:, 0], gmin, sp.cimax)
-    ci_uc = np.clip(ci_u[:, 1], gmin, sp.cimax)
-    gb = np.clip(su * 25.5, 0.1, 10.0)
-    rh12 = np.maximum(0.30, q12 / np.maximum(qsat(esat(t12), psurf), eps))
-    vub = np.where(exist[:, 0] > 0.5, vp[:, 0], np.where(exist[:, 2] > 0.5, vp[:, 2], vp[:, 4]))
-    vuc = np.where(exist[:, 3] > 0.5, vp[:, 3], vp[:, 5])
-
-    vmax = vub * tv3 * stresstu
-    rd = gam[0] * vub * tv3
-    ag_ub, an_ub = farquhar_c3(rad.topparu, ci_ub, vmax, gstar, kc, ko, sp.o2conc, rd, sp)
-    cs_ub, gs_ub, ci_ub = ball_berry(an_ub, cs_u[:, 0], gs_u[:, 0], ci_ub, co2, gb, rh12, cm[0],
-                                     cb[0], stresstu, gsmin[0], gmin, sp.cimax)
-    vmax = vuc * tv3 * stresstu
-    rd = gam[1] * vuc * tv3
-    ag_uc, an_uc = farquhar_c3(rad.topparu, ci_uc, vmax, gstar, kc, ko, sp.o2conc, rd, sp)
-    cs_uc, gs_uc, ci_uc = ball_berry(an_uc, cs_u[:, 1], gs_u[:, 1], ci_uc, co2, gb, rh12, cm[1],
-                                     cb[1], stresstu, gsmin[1], gmin, sp.cimax)
-    scale, a10su, a10du = canopy_scaling(rad.topparu, rad.termu6, rad.termu7, rad.scalcoefu,
-                                         lai[:, 1], sai[:, 1], a10[0], a10[1], dtime, eps)
-    agc_ub, agc_uc = ag_ub * scale, ag_uc * scale
-    anc_ub, anc_uc = an_ub * scale, an_uc * scale
-    csc = np.maximum(gmin, co2 - anc_ub / gb)
-    gsc_ub = np.maximum(np.maximum(gsmin[0], cb[0] * stresstu),
-                        cm[0] * anc_ub * rh12 / csc + cb[0] * stresstu)
-    csc = np.maximum(gmin, co2 - anc_uc / gb)
-    gsc_uc = np.maximum(np.maximum(gsmin[1], cb[1] * stresstu),
-                        cm[1] * anc_uc * rh12 / csc + cb[1] * stresstu)
-    tc_ub = total_conductance(su, gsc_ub)
-    tc_uc = total_conductance(su, gsc_uc)
-    dry = 1.0 - fwetu
-    agc_ub, agc_uc, anc_ub, anc_uc = agc_ub * dry, agc_uc * dry, anc_ub * dry, anc_uc * dry
-
-    kc, ko, gstar, tv3, tv4 = _temp_dep(tl, sp)
-    gmin = 1.05 * gstar
-    ci_ls = np.clip(ci_l[:, 0], gmin, sp.cimax)
-    ci_l3 = np.clip(ci_l[:, 1], gmin, sp.cimax)
-    ci_l4 = np.clip(ci_l[:, 2], 0.0, sp.cimax)
-    gb = np.clip(sl * 25.5, 0.1, 10.0)
-    rh34 = np.maximum(0.30, q34 / np.maximum(qsat(esat(t34), psurf), eps))
-    vls = vp[:, 8]
-    vmax = vls * tv3 * stresstl
-    rd = gam[2] * vls * tv3
-    ag_ls, an_ls = farquhar_c3(rad.topparl, ci_ls, vmax, gstar, kc, ko, sp.o2conc, rd, sp)
-    cs_ls, gs_ls, ci_ls = ball_berry(an_ls, cs_l[:, 0], gs_l[:, 0], ci_ls, co2, gb, rh34, cm[2],
-                                     cb[2], stresstl, gsmin[2], gmin, sp.cimax)
-    vl3 = vp[:, 11]
-    vmax = vl3 * tv3 * stresstl
-    rd = gam[4] * vl3 * tv3
-    ag_l3, an_l3 = farquhar_c3(rad.topparl, ci_l3, vmax, gstar, kc, ko, sp.o2conc, rd, sp)
-    cs_l3, gs_l3, ci_l3 = ball_berry(an_l3, cs_l[:, 1], gs_l[:, 1], ci_l3, co2, gb, rh34, cm[4],
-                                     cb[4], stresstl, gsmin[4], gmin, sp.cimax)
-    vl4 = vp[:, 10]
-    vmax = vl4 * tv4 * stresstl
-    rd = gam[3] * vl4 * tv4
-    ag_l4, an_l4 = farquhar_c4(rad.topparl, ci_l4, vmax, rd, sp)
-    cs_l4, gs_l4, ci_l4 = ball_berry(an_l4, cs_l[:, 2], gs_l[:, 2], ci_l4, co2, gb, rh34, cm[3],
-                                     cb[3], stresstl, gsmin[3], gmin, sp.cimax, c4=True)
-    scale, a10sl, a10dl = canopy_scaling(rad.topparl, rad.terml6, rad.terml7, rad.scalcoefl,
-                                         lai[:, 0], sai[:, 0], a10[2], a10[3], dtime, eps)
-    agc_ls, agc_l3, agc_l4 = ag_ls * scale, ag_l3 * scale, ag_l4 * scale
-    anc_ls, anc_l3, anc_l4 = an_ls * scale, an_l3 * scale, an_l4 * scale
-    csc_ls = np.maximum(gmin, co2 - anc_ls / gb)
-    csc_l3 = np.maximum(gmin, co2 - anc_l3 / gb)
-    csc_l4 = np.maximum(1.0e-8, co2 - anc_l4 / gb)
-    gsc_ls = np.maximum(np.maximum(gsmin[2], cb[2] * stresstl),
-                        cm[2] * anc_ls * rh34 / csc_ls + cb[2] * stresstl)
-    gsc_l3 = np.maximum(np.maximum(gsmin[4], cb[4] * stresstl),
-                        cm[4] * anc_l3 * rh34 / csc_l3 + cb[4] * stresstl)
-    gsc_l4 = np.maximum(np.maximum(gsmin[3], cb[3] * stresstl),
-                        cm[3] * anc_l4 * rh34 / csc_l4 + cb[3] * stresstl)
-    tc_ls = total_conductance(sl, gsc_ls)
-    tc_l3 = total_conductance(sl, gsc_l3)
-    tc_l4 = total_conductance(sl, gsc_l4)
-    dry = 1.0 - fwetl
-    agc_ls, agc_l3, agc_l4 = agc_ls * dry, agc_l3 * dry, agc_l4 * dry
-    anc_ls, anc_l3, anc_l4 = anc_ls * dry, anc_l3 * dry, anc_l4 * dry
+    sw = lambda k: bool(getattr(sp, k, False))
+    vraw = vmax_table(sp, n)
+    vp = vraw if vfac is None else vraw * vfac
+    vnom = select_vmax(exist, vp, frac, sw("vmax_mix"))
+    je = [None] * 5
+    if sw("acclim_light") and vfac is not None:
+        v0 = select_vmax(exist, vraw, frac, sw("vmax_mix"))
+        je = [vnom[k] / np.maximum(v0[k], eps) for k in range(5)]
+    cms = slopes(sp, frac, n)
+    gam, cb, gsmin = sp.gamma, sp.coefb, sp.gsmin
+    kn = getattr(sp, "cap_kn", 0.15)
+    leaf_only = sw("scale_leaf_only")
+    # (big leaf, Ball-Berry group, state column) of each canopy; C4 is the lower leaf 2
+    canopies = (
+        dict(leaves=((0, 0, 0), (1, 1, 1)), t=tu, tq=(t12, q12), s=su, par=rad.topparu,
+             t6=rad.termu6, t7=rad.termu7, sc=rad.scalcoefu, lv=lai[:, 1], sv=sai[:, 1],
+             stress=stresstu, fwet=fwetu, state=(ci_u, gs_u, cs_u), a10=(a10[0], a10[1]),
+             sunshade=sw("sunshade_u"), cap=sw("cap_scale_u"),
+             dbeam=getattr(rad, "dbeamu", None)),
+        dict(leaves=((2, 2, 0), (3, 4, 1), (4, 3, 2)), t=tl, tq=(t34, q34), s=sl,
+             par=rad.topparl, t6=rad.terml6, t7=rad.terml7, sc=rad.scalcoefl, lv=lai[:, 0],
+             sv=sai[:, 0], stress=stresstl, fwet=fwetl, state=(ci_l, gs_l, cs_l),
+             a10=(a10[2], a10[3]), sunshade=sw("sunshade_l"), cap=sw("cap_scale_l"),
+             dbeam=getattr(rad, "dbeaml", None)),
+    )
+    out = {}
+    a10_out = []
+    for cn in canopies:
+        kc, ko, gstar, tv3, tv4 = _temp_dep(cn["t"], sp)
+        gmin = 1.05 * gstar
+        gb = np.clip(cn["s"] * 25.5, 0.1, 10.0)
+        rh = np.maximum(0.30, cn["tq"][1] / np.maximum(qsat(esat(cn["tq"][0]), psurf), eps))
+        stress = cn["stress"]
+        ss = None
+        if cn["sunshade"]:
+            if cn["dbeam"] is None:
+                raise ValueError("sunshade needs the unscattered beam term of radiation")
+            ss = sunshade_canopy(cn["sc"], cn["dbeam"], cn["t6"], cn["t7"], cn["lv"], cn["sv"],
+                                 kn, eps)
+        leaves = []
+        for (k, grp, col) in cn["leaves"]:
+            c4 = k == 4
+            ci0 = np.clip(cn["state"][0][:, col], 0.0 if c4 else gmin, sp.cimax)
+            tv = tv4 if c4 else tv3
+            vmax = vnom[k] * tv * stress
+            rd = gam[grp] * vnom[k] * tv
+            if c4:
+                ag, an = farquhar_c4(cn["par"], ci0, vmax, rd, sp, je_fac=je[k])
+            else:
+                ag, an = farquhar_c3(cn["par"], ci0, vmax, gstar, kc, ko, sp.o2conc, rd, sp,
+                                     je_fac=je[k])
+            agc = anc = None
+            if ss is not None:
+                if c4:
+                    agc, anc = sunshade_c4(ss, ci0, vmax, rd, cn["lv"], sp, je_fac=je[k])
+                else:
+                    agc, anc = sunshade_c3(ss, ci0, vmax, gstar, kc, ko, sp.o2conc, rd, cn["lv"],
+                                           sp, je_fac=je[k])
+            cs, gs, ci = ball_berry(anc if ss is not None else an, cn["state"][2][:, col],
+                                    cn["state"][1][:, col], ci0, co2, gb, rh, cms[k], cb[grp],
+                                    stress, gsmin[grp], gmin, sp.cimax, c4=c4)
+            leaves.append(dict(k=k, grp=grp, c4=c4, ci0=ci0, vmax=vmax, rd=rd, ag=ag, an=an,
+                               agc=agc, anc=anc, cs=cs, gs=gs, ci=ci))
+        scale, a10s, a10d = canopy_scaling(cn["par"], cn["t6"], cn["t7"], cn["sc"], cn["lv"],
+                                           cn["sv"], cn["a10"][0], cn["a10"][1], dtime, eps,
+                                           leaf_only=leaf_only)
+        a10_out += [a10s, a10d]
+        fcap = capacity_integral(cn["lv"], kn) if (cn["cap"] and ss is None) else None
+        dry = 1.0 - cn["fwet"]
+        for lf in leaves:
+            k, grp, c4 = lf["k"], lf["grp"], lf["c4"]
+            if ss is not None:
+                agc, anc = lf["agc"], lf["anc"]
+            elif fcap is not None:
+                if c4:
+                    agc, anc = farquhar_c4(cn["par"], lf["ci0"], lf["vmax"], lf["rd"], sp,
+                                           f_light=scale, f_cap=fcap, je_fac=je[k])
+                else:
+                    agc, anc = farquhar_c3(cn["par"], lf["ci0"], lf["vmax"], gstar, kc, ko,
+                                           sp.o2conc, lf["rd"], sp, f_light=scale, f_cap=fcap,
+                                           je_fac=je[k])
+            else:
+                agc, anc = lf["ag"] * scale, lf["an"] * scale
+            csc = np.maximum(1.0e-8 if c4 else gmin, co2 - anc / gb)
+            gsc = np.maximum(np.maximum(gsmin[grp], cb[grp] * stress),
+                             cms[k] * anc * rh / csc + cb[grp] * stress)
+            lf["tc"] = total_conductance(cn["s"], gsc)
+            lf["agc"], lf["anc"] = agc * dry, anc * dry
+            out[k] = lf
     st = np.stack
+    ub, uc, ls, l3, l4 = (out[k] for k in range(5))
     return SimpleNamespace(
-        ci_upper=st([ci_ub, ci_uc], 1), gs_upper=st([gs_ub, gs_uc], 1),
-        cs_upper=st([cs_ub, cs_uc], 1), an_upper=st([anc_ub, anc_uc], 1),
-        ag_upper=st([agc_ub, agc_uc], 1),
-        ci_lower=st([ci_ls, ci_l3, ci_l4], 1), gs_lower=st([gs_ls, gs_l3, gs_l4], 1),
-        cs_lower=st([cs_ls, cs_l3, cs_l4], 1), an_lower=st([anc_ls, anc_l3, anc_l4], 1),
-        ag_lower=st([agc_ls, agc_l3, agc_l4], 1),
-        totcondub=tc_ub, totconduc=tc_uc, totcondls=tc_ls, totcondl3=tc_l3, totcondl4=tc_l4,
-        a10=(a10su, a10du, a10sl, a10dl))
+        ci_upper=st([ub["ci"], uc["ci"]], 1), gs_upper=st([ub["gs"], uc["gs"]], 1),
+        cs_upper=st([ub["cs"], uc["cs"]], 1), an_upper=st([ub["anc"], uc["anc"]], 1),
+        ag_upper=st([ub["agc"], uc["agc"]], 1),
+        ci_lower=st([ls["ci"], l3["ci"], l4["ci"]], 1),
+        gs_lower=st([ls["gs"], l3["gs"], l4["gs"]], 1),
+        cs_lower=st([ls["cs"], l3["cs"], l4["cs"]], 1),
+        an_lower=st([ls["anc"], l3["anc"], l4["anc"]], 1),
+        ag_lower=st([ls["agc"], l3["agc"], l4["agc"]], 1),
+        totcondub=ub["tc"], totconduc=uc["tc"], totcondls=ls["tc"], totcondl3=l3["tc"],
+        totcondl4=l4["tc"], a10=tuple(a10_out))

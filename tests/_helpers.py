@@ -79,6 +79,46 @@ def host_step(f, slot, prec, bedrock=True, consts=None, params=None, dtime=3600.
     return abi.unpack(o, n)
 
 
+def host_step_ext(f, slot, prec, params, aux=None, bedrock=True, consts=None, dtime=3600.0,
+                  glob=None):
+    """One hour of the kernel source's ibis_step_ext path built for the host.  ``aux``:
+    optional per-cell ``seasonal.acclim_x`` / ``atm.td`` arrays; a per-cell stressfac is
+    ``params.stomata_params.stressfac`` as an (n,) array."""
+    from ibis_amd.step import slot_tables
+    lib = C.CDLL(HOSTCHECK)
+    n = len(f["soil.tg"])
+    consts = consts or get_constants()
+    g = glob or glob_of()
+    p = abi.params_from(consts, params, g["snow"], g["hsoi"], g["froot"], g["co2conc"])
+    dt = np.float64 if prec == "f64" else np.float32
+    a = abi.pack(f, n, dtype=dt)
+    o = a.copy()
+    s = abi.pack_slots(slot, n)
+    tab = slot_tables(params)
+    aux = aux or {}
+    keep = []
+
+    def plane(v):
+        if v is None:
+            return 0
+        arr = np.ascontiguousarray(np.asarray(v, dtype=np.float64).astype(dt))
+        keep.append(arr)
+        return arr.ctypes.data
+    sp = params.stomata_params
+    x = abi.stomata_ext(sp, tab.ctypes.data, plane(aux.get("seasonal.acclim_x")),
+                        plane(aux.get("atm.td")),
+                        plane(sp.stressfac if np.ndim(sp.stressfac) else None))
+    if "seasonal.acclim_x" not in aux:
+        x.switches &= ~abi.SW_ACCLIM
+    flags = PREC_CODE[prec] | (abi.FLAG_BEDROCK if bedrock else 0)
+    rc = lib.ibis_hostcheck_step_ext(a.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p),
+                                     C.c_int64(n), C.c_int64(n), s.ctypes.data_as(C.c_void_p),
+                                     C.c_int64(n), C.byref(p), C.byref(x), C.c_double(dtime),
+                                     C.c_double(g["nytimes"]), C.c_int(flags))
+    assert rc == 0
+    return abi.unpack(o, n)
+
+
 def failing_cells(got, refs, keys, tol):
     return compare.failing(got, refs, keys, tol)
 

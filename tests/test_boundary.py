@@ -77,10 +77,10 @@ def test_argument_errors_return_codes_without_touching_the_gpu():
 
 def test_workspace_size_per_precision():
     lib = _lib.load()
-    # 34 planes in the solve precision, 25 in the bulk precision
-    assert lib.ibis_step_workspace_bytes(100, abi.PREC_MIXED) == 100 * (34 * 8 + 25 * 4)
-    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F32) == 100 * 59 * 4
-    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F64) == 100 * 59 * 8
+    # 34 planes in the solve precision, 27 in the bulk precision
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_MIXED) == 100 * (34 * 8 + 27 * 4)
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F32) == 100 * 61 * 4
+    assert lib.ibis_step_workspace_bytes(100, abi.PREC_F64) == 100 * 61 * 8
     assert lib.ibis_step_workspace_bytes(-1, 0) < 0
 
 

@@ -96,6 +96,29 @@ def test_inplace_equals_functional():
     assert new.nytimes == st2.nytimes == 1.0
 
 
+def test_registered_operators_are_the_step():
+    """torch.ops.ibis.step_ (in place) and torch.ops.ibis.step (functional, the path of
+    timestep_body) give the direct launch's arena bit for bit."""
+    from ibis_amd import abi, ops  # noqa: F401
+    from ibis_amd import step as S
+    n = 1500
+    f, slot = workload(n, seed=4, hour=11, prec="mixed")
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda")
+    c, p = get_constants(), default_sim_params()
+    ref = st.clone()
+    timestep_(ref, c, p, 3600.0)
+    a = st.clone()
+    code = abi.PREC_MIXED
+    ws = S.workspace(a, code, torch.cuda.current_stream())
+    args = (S.type_table(p, a.device), abi.params_bytes(S.ibis_params(a, c, p)), 3600.0, 0.0,
+            n, code | abi.FLAG_BEDROCK)
+    torch.ops.ibis.step_(a.arena, ws, a.slot_type, *args)
+    out = torch.ops.ibis.step(st.arena, ws, st.slot_type, *args)
+    torch.cuda.synchronize()
+    assert torch.equal(a.arena, ref.arena)
+    assert torch.equal(out, ref.arena)
+
+
 def test_functional_step_leaves_input_untouched():
     n = 1000
     f, slot = workload(n, seed=4, hour=9, prec="mixed")
