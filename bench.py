@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-hours", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stomata", default="f77", choices=["f77", "release"],
+                    help="f77: every optional StomataParams switch off (the default parameter "
+                         "file); release: the switches of the release parameter file, a "
+                         "per-cell stressfac and a seasonal acclimation state (ibis_step_ext)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the hours, copy a buffer of this many MiB once (a kernel of "
                          "known traffic that calibrates the HBM byte counters of a profile)")
@@ -82,9 +86,11 @@ def main():
     dtype = torch.float64 if args.precision == "f64" else torch.float32
     pitch = (n + 255) // 256 * 256
     fields, slot, lat = synth.make_cells(n, args.seed)
+    consts, params = get_constants(), default_sim_params(bedrock=True)
+    if args.stomata == "release":
+        release_inputs(params, fields, n, args.seed)
     st = IBISState.from_numpy(fields, slot, synth.HSOI, root_fractions(), device=dev,
                               dtype=dtype, pitch=pitch)
-    consts, params = get_constants(), default_sim_params(bedrock=True)
     cparams = S.ibis_params(st, consts, params)
 
     atm0 = L.PLANE_OFFSET["atm.ta"]
@@ -147,27 +153,42 @@ def main():
         "data": "synthetic (seeded cells and hourly forcing, benchmark distributions)",
         "config": {"workload": f"plain hourly step, {n:,} cells x {args.steps} h, one launch per hour",
                    "cells": n, "hours": args.steps, "precision": args.precision,
-                   "soil": "4 m soil + 8 m bedrock", "plant_type_slots": 12},
+                   "soil": "4 m soil + 8 m bedrock", "plant_type_slots": 12,
+                   "stomata": args.stomata},
         "kernel_ms_mean": kmean * 1e3, "kernel_ms_min": float(np.min(kms)),
         "finite_fraction": finite,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": profiled_traffic(args.precision, n),
+                     "traffic": (profiled_traffic(args.precision, n)
+                                 if args.stomata == "f77" else None),
                      "bytes_per_cell_hour": per_cell,
-                     "scope": "one ibis_step call: surface + canopy + ground kernels"},
+                     "scope": ("one ibis_step call" if args.stomata == "f77" else
+                               "one ibis_step_ext call") + ": surface + canopy + ground kernels"},
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(n, args.cpu_hours, args.seed)
+        res["cpu_baseline"] = cpu_baseline(n, args.cpu_hours, args.seed, args.stomata)
     print(json.dumps(res), flush=True)
 
 
-def cpu_baseline(n, hours, seed):
+def release_inputs(params, fields, n, seed):
+    """The release switches on ``params``, with a per-cell stressfac and a seasonal
+    acclimation state drawn for the cells (added to ``fields``)."""
+    from ibis_amd.params import release_switches
+    release_switches(params)
+    rng = np.random.default_rng(seed + 1)
+    params.stomata_params.stressfac = rng.uniform(-9.0, -3.0, n)
+    fields["seasonal.acclim_x"] = rng.uniform(-15.0, 25.0, n)
+
+
+def cpu_baseline(n, hours, seed, stomata="f77"):
     """The oracle (NumPy FP64, one core) on the first ``hours`` hours of the same cells."""
     from ibis_amd import synth
     from ibis_amd.params import default_sim_params, get_constants
     from oracle import params as OP, run as orun
     fields, slot, lat = synth.make_cells(n, seed)
     consts, params = get_constants(), default_sim_params(bedrock=True)
+    if stomata == "release":
+        release_inputs(params, fields, n, seed)
     st = dict(fields)
     t = 0.0
     for h in range(hours):

@@ -90,6 +90,9 @@ int ibis_day_begin(const IbisArena* state, const IbisArena* day, const IbisArena
   using namespace ibis;
   const int rc = day_check(state, day, daily, acc, p, ncell, flags, "ibis_day_begin");
   if (rc != IBIS_OK) return rc;
+  if (p->seasonal && (!p->lat_rad || !(p->acclim_tau != 0.0)))
+    return fail(IBIS_E_ARG, "ibis_day_begin: the seasonal states need lat_rad and a nonzero "
+                            "acclim_tau");
   if (ncell == 0) return IBIS_OK;
   const dim3 grid((unsigned)((ncell + kDayBlock - 1) / kDayBlock)), block(kDayBlock);
   hipStream_t s = (hipStream_t)hip_stream;

@@ -42,24 +42,54 @@ IBIS_HD IBIS_INLINE S drop_update(S x, S anc, S stress, const IbisDayParams& p, 
              : mn(S(1), x + ddfac);
 }
 
+// Astronomical day length (h) at latitude lat (radians) on day of year doy
+// (state/seasonal.py:53-58)
+template <class S> IBIS_HD IBIS_INLINE S daylength_hours(S lat, S doy) {
+  const S decl = S(0.40910518) * r_sin(S(2.0 * 3.14159265358979323846) * (doy - S(81)) / S(365));
+  const S x = clip(-r_tan(lat) * r_tan(decl), S(-1), S(1));
+  return S(24.0 / 3.14159265358979323846) * r_acos(x);
+}
+
+// Seasonal states, advanced once a day before phenology (loops_gswp3.py:418-440); returns
+// the senescence triggers of the upper and lower canopy.
+template <class S>
+IBIS_HD IBIS_INLINE void seasonal_update(const DayView<S>& v, const IbisDayParams& p, S td,
+                                         bool& sen_u, bool& sen_l) {
+  S* se = (S*)p.seasonal;
+  S& ax = se[v.i];
+  S& su = se[v.dp + v.i];
+  S& sl = se[2 * v.dp + v.i];
+  const S tc = td - S(273.16);
+  const S lat = S(p.lat_rad[v.i]);
+  const S dl = daylength_hours(lat, S(p.jday));
+  const bool dec = dl < daylength_hours(lat, S(p.jday - 1));
+  ax = ax + (tc - ax) / S(p.acclim_tau);
+  su = (dec && dl < S(p.sen_pstart_u)) ? su + mx(S(p.sen_tb_u) - tc, S(0)) : S(0);
+  sl = (dec && dl < S(p.sen_pstart_l)) ? sl + mx(S(p.sen_tb_l) - tc, S(0)) : S(0);
+  sen_u = p.sen_photoperiod_u && su >= S(p.sen_ycrit_u);
+  sen_l = p.sen_photoperiod_l && sl >= S(p.sen_ycrit_l);
+}
+
 template <class S> IBIS_HD IBIS_INLINE void day_begin_cell(const DayView<S>& v, const IbisDayParams& p) {
   const S tmelt = S(p.tmelt), eps = S(p.epsilon);
   const S ddfac = S(1.0 / p.ddays);
   const S td = v.td();
   const S a10td = v.d(IBIS_DAY_A10TD);
+  bool sen_u = false, sen_l = false;
+  if (p.seasonal) seasonal_update(v, p, td, sen_u, sen_l);
   // cold phenology, upper canopy
   const S tthr_u = mx(mx(S(0) + tmelt, v.d(IBIS_DAY_TC) + S(p.tthresh_u_offset) + tmelt),
                       S(p.tthresh_u_abs) + tmelt);
   const S gthr_u = S(p.gthresh_u_c) + tmelt;
-  const S agddu = (a10td < gthr_u) ? S(0) : v.d(IBIS_DAY_AGDDU) + td - gthr_u;
-  const S tempu = (a10td < tthr_u)
+  const S agddu = (a10td < gthr_u || sen_u) ? S(0) : v.d(IBIS_DAY_AGDDU) + td - gthr_u;
+  const S tempu = (a10td < tthr_u || sen_u)
                       ? mx(S(0), v.d(IBIS_DAY_TEMPU) - ddfac)
                       : mn(S(1), mx(S(0), agddu - S(p.gdd_thresh_u)) / S(p.gdd_ramp_u));
   // lower canopy
   const S tthr_l = S(p.tthresh_l_c) + tmelt;
   const S gthr_l = S(p.gthresh_l_c) + tmelt;
-  const S agddl = (a10td < gthr_l) ? S(0) : v.d(IBIS_DAY_AGDDL) + td - gthr_l;
-  const S templ = (a10td < tthr_l)
+  const S agddl = (a10td < gthr_l || sen_l) ? S(0) : v.d(IBIS_DAY_AGDDL) + td - gthr_l;
+  const S templ = (a10td < tthr_l || sen_l)
                       ? mx(S(0), v.d(IBIS_DAY_TEMPL) - ddfac)
                       : mn(S(1), mx(S(0), agddl - S(p.gdd_thresh_l)) / S(p.gdd_ramp_l));
   // drought phenology

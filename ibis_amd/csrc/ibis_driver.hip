@@ -141,6 +141,9 @@ int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisAre
     return fail(IBIS_E_FLAGS, "ibis_accumulate: unknown flag bits or precision");
   if (!(p->dtime > 0.0) || !(p->ndaypy > 0.0))
     return fail(IBIS_E_ARG, "ibis_accumulate: dtime and ndaypy must be positive");
+  if (p->split_by_capacity && (!p->vmax_table || (p->slot_type && p->slot_pitch < ncell)))
+    return fail(IBIS_E_ARG, "ibis_accumulate: the capacity split needs the Vmax table and "
+                            "slot_pitch >= ncell");
   if (ncell == 0) return IBIS_OK;
   const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
   hipStream_t s = (hipStream_t)hip_stream;

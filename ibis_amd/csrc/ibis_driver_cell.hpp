@@ -209,7 +209,25 @@ template <class R> struct AccIn {
   R lai_l, lai_u, fu, fl, fi, ts, tsoi[IBIS_NSOILAY];
   R frac[IBIS_NPFT], cbiow[IBIS_NPFT], cbior[IBIS_NPFT], sapfrac, tco2mic;
   R raina, snowa, coszen, ginvap, gsuvap, gtrans, grunof, gdrain, fvapi;
+  R vmax[IBIS_NPFT];        // Vmax of each slot (split_by_capacity only)
 };
+
+// NPP shares of the plant types of one big leaf (bit j of mask): displayed leaf area x own
+// Vmax, rescaled to the leaf-area total; the leaf-area shares where no capacity
+// (python-port/ibis/physics/stomata.py:574-584)
+template <class R>
+IBIS_HD IBIS_INLINE void capacity_shares(R share[IBIS_NPFT], const R frac[IBIS_NPFT],
+                                         const R vmax[IBIS_NPFT], int mask) {
+  R tw = R(0), tf = R(0);
+  for (int j = 0; j < IBIS_NPFT; ++j)
+    if (mask & (1 << j)) {
+      tw = tw + frac[j] * vmax[j];
+      tf = tf + frac[j];
+    }
+  const R eps = R(1.0e-12);
+  for (int j = 0; j < IBIS_NPFT; ++j)
+    if (mask & (1 << j)) share[j] = tw > eps ? frac[j] * vmax[j] / mx(tw, eps) * tf : frac[j];
+}
 
 // The step's increment of each accumulator (same order as IbisAcc).
 template <class R>
@@ -226,7 +244,15 @@ IBIS_HD IBIS_INLINE void acc_increments(R inc[IBIS_NACC], const AccIn<R>& a,
   const R c_fac = R(12.0e-3) * dt;
   const R per_s = R(1.0 / (p.ndaypy * 86400.0) * (1000.0 / 12.0));
   // big leaf feeding each PFT: upper broadleaf 0 1 2 4 6, upper conifer 3 5 7, shrubs 8 9,
-  // C4 grass 10, C3 grass 11
+  // C4 grass 10, C3 grass 11; its NPP shared by leaf area, or leaf area x Vmax
+  R share[IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) share[j] = a.frac[j];
+  if (p.split_by_capacity) {
+    capacity_shares(share, a.frac, a.vmax, (1 << 0) | (1 << 1) | (1 << 2) | (1 << 4) | (1 << 6));
+    capacity_shares(share, a.frac, a.vmax, (1 << 3) | (1 << 5) | (1 << 7));
+    capacity_shares(share, a.frac, a.vmax, (1 << 8) | (1 << 9));
+  }
   R npp_step = R(0);
 #pragma unroll
   for (int j = 0; j < IBIS_NPFT; ++j) {
@@ -236,8 +262,8 @@ IBIS_HD IBIS_INLINE void acc_increments(R inc[IBIS_NACC], const AccIn<R>& a,
     if (upper) an = conifer ? a.an_u[1] : a.an_u[0];
     else if (j < 10) an = a.an_l[0];
     else an = (j == 10) ? a.an_l[2] : a.an_l[1];
-    const R gpp = upper ? an * a.frac[j] * a.lai_u * a.fu
-                        : an * a.frac[j] * a.lai_l * a.fl * smask;
+    const R sh = j < 10 ? share[j] : a.frac[j];
+    const R gpp = upper ? an * sh * a.lai_u * a.fu : an * sh * a.lai_l * a.fl * smask;
     const R rw = R(p.rwood[j]) * per_s, rr = R(p.rroot[j]) * per_s;
     R resp;
     if (upper) resp = rw * a.cbiow[j] * a.sapfrac * funca + rr * a.cbior[j] * funcb;
@@ -329,6 +355,12 @@ IBIS_HD IBIS_INLINE void accumulate_cell(const S* st, int64_t pitch, const S* vx
   a.grunof = ld(IBIS_P_HYDROLOGY_GRUNOF);
   a.gdrain = ld(IBIS_P_HYDROLOGY_GDRAIN);
   a.fvapi = ld(IBIS_P_CANOPY_FVAPI);
+  if (p.split_by_capacity) {
+    for (int j = 0; j < IBIS_NPFT; ++j) {
+      const int s = p.slot_type ? (int)p.slot_type[(int64_t)j * p.slot_pitch + i] : -1;
+      a.vmax[j] = S(p.vmax_table[(s >= 0 && s < IBIS_NTYPE) ? s : IBIS_NTYPE + j]);
+    }
+  }
   S inc[IBIS_NACC];
   acc_increments(inc, a, p);
 #pragma unroll

@@ -27,6 +27,12 @@ template <class T> struct is_dual<Dual<T>> { static constexpr bool value = true;
 IBIS_HD IBIS_INLINE float r_exp(float x) { return expf(x); }
 IBIS_HD IBIS_INLINE double r_exp(double x) { return exp(x); }
 IBIS_HD IBIS_INLINE float r_expm1(float x) { return expm1f(x); }
+IBIS_HD IBIS_INLINE float r_sin(float x) { return sinf(x); }
+IBIS_HD IBIS_INLINE double r_sin(double x) { return sin(x); }
+IBIS_HD IBIS_INLINE float r_tan(float x) { return tanf(x); }
+IBIS_HD IBIS_INLINE double r_tan(double x) { return tan(x); }
+IBIS_HD IBIS_INLINE float r_acos(float x) { return acosf(x); }
+IBIS_HD IBIS_INLINE double r_acos(double x) { return acos(x); }
 IBIS_HD IBIS_INLINE double r_expm1(double x) { return expm1(x); }
 IBIS_HD IBIS_INLINE float r_log(float x) { return logf(x); }
 IBIS_HD IBIS_INLINE double r_log(double x) { return log(x); }

@@ -49,7 +49,8 @@ assert NDAYPLANE == 59
 
 @dataclass
 class PhenoParams:
-    """physics/phenology.py:20-85 (thresholds; the seasonal-state switches are off)."""
+    """physics/phenology.py:20-86: thresholds, and the photoperiod senescence of the optional
+    seasonal states (off by default)."""
     tthresh_u_offset: float = 5.0
     gthresh_u_c: float = 0.0
     gdd_thresh_u: float = 100.0
@@ -63,6 +64,18 @@ class PhenoParams:
     ddays: float = 15.0
     drop_stress_thresh: float = -1.0
     tthresh_u_abs: float = -100.0
+    sen_photoperiod_u: bool = False
+    sen_pstart_u: float = 14.0
+    sen_tb_u: float = 20.0
+    sen_ycrit_u: float = 300.0
+    sen_photoperiod_l: bool = False
+    sen_pstart_l: float = 14.0
+    sen_tb_l: float = 20.0
+    sen_ycrit_l: float = 300.0
+
+
+#: the seasonal states (python-port/ibis/state/seasonal.py:36-43), planes of a (3, pitch) block
+SEASONAL_FIELDS = ("acclim_x", "ssen_u", "ssen_l")
 
 
 @dataclass
@@ -114,7 +127,8 @@ class BGCParams:
     k_scale: float = 1.0
 
 
-_PH = [f.name for f in fields(PhenoParams)]
+_PH = [f.name for f in fields(PhenoParams) if not f.name.startswith("sen_")]
+_SEN = ("sen_pstart_u", "sen_tb_u", "sen_ycrit_u", "sen_pstart_l", "sen_tb_l", "sen_ycrit_l")
 _BG = [f.name for f in fields(BGCParams)]
 
 
@@ -123,15 +137,25 @@ class IbisDayParams(C.Structure):
                 + [(n, C.c_double) for n in _BG[:8]] + [("cnr", C.c_double * 8)]
                 + [(n, C.c_double) for n in _BG[9:]]
                 + [("hsoi", C.c_double * (L.NSOILAY + 1)), ("ndaypy", C.c_double),
-                   ("day_sec", C.c_double), ("nsoilcall", C.c_int32), ("pad0", C.c_int32)])
+                   ("day_sec", C.c_double), ("nsoilcall", C.c_int32), ("pad0", C.c_int32),
+                   ("acclim_tau", C.c_double)] + [(n, C.c_double) for n in _SEN]
+                + [("sen_photoperiod_u", C.c_int32), ("sen_photoperiod_l", C.c_int32),
+                   ("jday", C.c_int32), ("pad1", C.c_int32), ("seasonal", C.c_void_p),
+                   ("lat_rad", C.c_void_p)])
 
 
 def day_params(hsoi, pheno: PhenoParams | None = None, bgc: BGCParams | None = None,
-               epsilon=1.0e-7, tmelt=273.16, nsoilcall=1, ndaypy=365.0, day_sec=86400.0):
+               epsilon=1.0e-7, tmelt=273.16, nsoilcall=1, ndaypy=365.0, day_sec=86400.0,
+               acclim_tau=10.93):
+    """IbisDayParams; the seasonal states stay off until ``attach_seasonal`` sets their
+    block and latitudes (and the day loop its day of year)."""
     pheno, bgc = pheno or PhenoParams(), bgc or BGCParams()
     p = IbisDayParams()
-    for n in _PH:
+    for n in _PH + list(_SEN):
         setattr(p, n, float(getattr(pheno, n)))
+    p.sen_photoperiod_u = 1 if pheno.sen_photoperiod_u else 0
+    p.sen_photoperiod_l = 1 if pheno.sen_photoperiod_l else 0
+    p.acclim_tau = float(acclim_tau)
     for n in _BG:
         if n == "cnr":
             p.cnr[:] = [float(x) for x in bgc.cnr]
@@ -141,6 +165,26 @@ def day_params(hsoi, pheno: PhenoParams | None = None, bgc: BGCParams | None = N
     p.hsoi[:] = [float(x) for x in np.asarray(hsoi, dtype=np.float64)[:L.NSOILAY + 1]]
     p.ndaypy, p.day_sec, p.nsoilcall = float(ndaypy), float(day_sec), int(nsoilcall)
     return p
+
+
+def attach_seasonal(state: IBISState, p: IbisDayParams, lat_deg, acclim_x0: float = 0.0):
+    """Give ``state`` the seasonal states (state/seasonal.py:46-50: acclimation state
+    ``acclim_x0`` deg C, senescence sums zero) as a (3, pitch) device block whose planes are the
+    state's ``seasonal.*`` leaves, and point ``p`` at it and at the cells' latitudes.  Returns
+    the block."""
+    block = torch.zeros((3, state.pitch), dtype=state.dtype, device=state.device)
+    block[0, :state.n] = acclim_x0
+    for j, k in enumerate(SEASONAL_FIELDS):
+        old = state.aux.get(f"seasonal.{k}")
+        if old is not None:
+            block[j].copy_(old)
+        state.aux[f"seasonal.{k}"] = block[j]
+    lat = np.zeros(state.pitch)
+    lat[:state.n] = np.deg2rad(np.asarray(lat_deg, dtype=np.float64))
+    lat_t = torch.as_tensor(lat, device=state.device)
+    p.seasonal, p.lat_rad = block.data_ptr(), lat_t.data_ptr()
+    p._keep = (block, lat_t)
+    return block
 
 
 def _lib_daily():
@@ -278,6 +322,8 @@ def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR
     a = _arena(state.arena)
     for k, w in enumerate(wdays):
         d = day0 + k
+        p.jday = int(weather.jday0 + d)
+        state.aux["atm.td"] = w[0]            # the day's mean temperature (frost ramp)
         day_begin(state, day, w, acc, p, stream=st)
         wa = _arena(w)
         for h in range(nstep):

@@ -43,7 +43,10 @@ DAILY_NAMES = ("td", "tmax", "tmin", "qd", "ud", "precip", "cloud", "psurf", "pl
 class IbisAccParams(C.Structure):
     _fields_ = [("rwood", C.c_double * L.NPFT), ("rroot", C.c_double * L.NPFT),
                 ("rgrowth", C.c_double * L.NPFT), ("froot", (C.c_double * 2) * L.NSOILAY),
-                ("dtime", C.c_double), ("ndaypy", C.c_double)]
+                ("dtime", C.c_double), ("ndaypy", C.c_double),
+                ("split_by_capacity", C.c_int32), ("pad0", C.c_int32),
+                ("slot_type", C.c_void_p), ("slot_pitch", C.c_int64),
+                ("vmax_table", C.c_void_p)]
 
 
 @dataclass
@@ -147,10 +150,12 @@ class DiurnalForcing:
 
 class Accumulators:
     """Per-step sums of the reference's driver with Kahan compensation.  ``vegx``: dict with
-    cbiow (N, 12), cbior (N, 12), sapfrac (N,), tco2mic (N,) (zeros when omitted)."""
+    cbiow (N, 12), cbior (N, 12), sapfrac (N,), tco2mic (N,) (zeros when omitted).  With
+    ``params.stomata_params.gpp_split_by_capacity`` the NPP of each big leaf is shared among
+    its plant types by leaf area x Vmax (the slot types and Vmax table of ``params``)."""
 
     def __init__(self, state: IBISState, vegx: dict | None = None, dtime: float = 3600.0,
-                 resp: RespirationParams | None = None):
+                 resp: RespirationParams | None = None, params=None):
         n, pitch = state.n, state.pitch
         host = np.zeros((NVEGX, pitch))
         if vegx:
@@ -162,6 +167,14 @@ class Accumulators:
         self.acc = torch.zeros((2 * NACC, pitch), dtype=state.dtype, device=state.device)
         self.params = acc_params(state, dtime, resp)
         self.n = n
+        self._vmax = None
+        if params is not None and getattr(params.stomata_params, "gpp_split_by_capacity", False):
+            self._vmax = S.type_table(params, state.device)
+            self.params.split_by_capacity = 1
+            self.params.vmax_table = self._vmax.data_ptr()
+            if state.slot_type is not None:
+                self.params.slot_type = state.slot_type.data_ptr()
+                self.params.slot_pitch = state.pitch
 
     def zero_(self, names=None):
         if names is None:

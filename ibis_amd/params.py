@@ -329,3 +329,21 @@ def load_params_yaml(path: str, bedrock: bool = True) -> SimParams:
     import yaml
     with open(path) as fh:
         return sim_params_from_dict(yaml.safe_load(fh) or {}, bedrock=bedrock)
+
+
+def release_switches(sim: SimParams) -> SimParams:
+    """Set the switches of the reference's release parameter file on ``sim`` (in place):
+    sunlit/shaded canopies, composition-weighted Vmax and slopes, acclimation with constants
+    per plant type (types 3, 5, 8), the capacity-weighted GPP split and the consistent soil
+    water weights (python-port/configs/params/params_31s_ch4h.yaml:55-90, :299).  The
+    calibrated values themselves come from the file (``load_params_yaml``)."""
+    sp = sim.stomata_params
+    sp.acclim_on = sp.acclim_per_pft = True
+    sp.acclim_pfts = (3, 5, 8)
+    sp.sunshade_u = sp.sunshade_l = True
+    sp.cap_kn = 0.15
+    sp.coefm_mix = sp.vmax_mix = True
+    sp.gpp_split_by_capacity = True
+    sp.acclim_light = sp.frost_on = False
+    sim.h2o_consistent_weights = True
+    return sim

@@ -19,8 +19,9 @@
  * Day arena: IBIS_NDAYPLANE planes of `pitch` cells in the state's element type (float for
  * IBIS_PREC_MIXED / F32, double for F64), fields below.  The day's mean temperature is plane
  * 0 (td) of the daily-weather arena of ibis_forcing_diurnal.  Default configuration of the
- * reference: scalar Lloyd-Taylor E0, no SOM-specific sensitivity, no seasonal states or
- * photoperiod senescence, positional plant-type habits.  Stream-ordered; allocates nothing.
+ * reference: scalar Lloyd-Taylor E0, no SOM-specific sensitivity, positional plant-type
+ * habits; the seasonal states and photoperiod senescence are optional (IbisDayParams).
+ * Stream-ordered; allocates nothing.
  */
 #ifndef IBIS_DAILY_H
 #define IBIS_DAILY_H
@@ -69,6 +70,19 @@ typedef struct IbisDayParams {
   double hsoi[IBIS_NSOILAY + 1];
   double ndaypy, day_sec;
   int32_t nsoilcall, pad0;
+  /* Seasonal states (python-port/ibis/state/seasonal.py:36-58; loops_gswp3.py:418-440).
+   * With `seasonal` set, day begin first advances them from the day's mean temperature T:
+   * the acclimation state S += (T - S) / acclim_tau, and per canopy the senescence sum, which
+   * adds max(sen_tb - T, 0) while day length (from lat_rad and jday) is decreasing and below
+   * sen_pstart and resets otherwise; with sen_photoperiod_u / _l a sum >= sen_ycrit forces the
+   * canopy's cold branch and degree-day reset (physics/phenology.py:148-150, :170-172).
+   * seasonal: 3 planes of the day arena's pitch (acclim_x deg C, ssen_u, ssen_l deg C day) in
+   * the state's element type, device memory; lat_rad: ncell doubles, device memory. */
+  double acclim_tau, sen_pstart_u, sen_tb_u, sen_ycrit_u, sen_pstart_l, sen_tb_l, sen_ycrit_l;
+  int32_t sen_photoperiod_u, sen_photoperiod_l;
+  int32_t jday, pad1;        /* day of year of this day, 1-based */
+  void* seasonal;
+  const double* lat_rad;
 } IbisDayParams;
 
 /* Day begin.  state: hourly-step arena (vegetation frac / lai / ztop / zbot written, fu fl

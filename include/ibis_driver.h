@@ -77,6 +77,15 @@ typedef struct IbisAccParams {
   double froot[IBIS_NSOILAY][2];
   double dtime;
   double ndaypy;              /* days per year of the respiration rates */
+  /* StomataParams.gpp_split_by_capacity (loops_gswp3.py:197-206): with split_by_capacity
+   * set, each upper or shrub big leaf's NPP is shared among its plant types by displayed
+   * leaf area x own Vmax (renormalised to the leaf-area total) instead of leaf area alone.
+   * The Vmax of each slot comes from the plant-type table as in ibis_step: slot_type (NULL:
+   * every slot generic) and vmax_table (IBIS_TYPE_ROWS doubles), device memory. */
+  int32_t split_by_capacity, pad0;
+  const int8_t* slot_type;
+  int64_t slot_pitch;
+  const double* vmax_table;
 } IbisAccParams;
 
 /* Add one step to the accumulators (python-port/ibis/timestepping/loops_gswp3.py:161-302):

@@ -13,8 +13,9 @@ its hourly steps (python-port/ibis/timestepping/loops_gswp3.py:382-583):
               heterotrophic and root respiration added to the annual sums (:577-585).
 
 Configuration restated: the default one (scalar lt_e0, wfps_opt and k_scale; SOM sensitivity
-off, so the slow and passive pools take the litter factor; no seasonal states, no
-photoperiod senescence, no per-type habits, no stress accumulator).
+off, so the slow and passive pools take the litter factor; no per-type habits, no stress
+accumulator), and the optional seasonal states with photoperiod senescence
+(``seasonal_update``; state/seasonal.py, loops_gswp3.py:418-440).
 
 Pins (tests/test_oracle_daily.py): closed forms of the litter splits and of the climate
 factors at their reference points; the carbon balance that defines tco2mic; the reference's
@@ -22,7 +23,11 @@ own qualitative checks (tests/test_biogeochem.py:259-405, tests/test_f77_verific
 693-735): cold lowers tempu, warmth raises agddu, negative assimilation drops leaves, LAI
 capped at 12, fractions of each canopy sum to one, pools stay non-negative, nitrogen
 mineralisation non-negative, 30 days stay finite; and the brute-force equality of one call
-with nsoilcall = 2 against two calls.
+with nsoilcall = 2 against two calls.  The seasonal states by the reference's
+tests/test_seasonal_formulations.py: day length 12 h at the equinox and the polar limits,
+the acclimation state's first-order relaxation to a constant temperature, senescence sums
+accumulating only while days shorten below Pstart and resetting otherwise, the trigger
+forcing leaf fall.
 """
 from __future__ import annotations
 
@@ -53,6 +58,15 @@ class PhenoParams:
     ddays: float = 15.0
     drop_stress_thresh: float = -1.0
     tthresh_u_abs: float = -100.0
+    # photoperiod-gated senescence (physics/phenology.py:79-86; state/seasonal.py:16-24)
+    sen_photoperiod_u: bool = False
+    sen_pstart_u: float = 14.0
+    sen_tb_u: float = 20.0
+    sen_ycrit_u: float = 300.0
+    sen_photoperiod_l: bool = False
+    sen_pstart_l: float = 14.0
+    sen_tb_l: float = 20.0
+    sen_ycrit_l: float = 300.0
 
 
 @dataclass
@@ -104,24 +118,59 @@ class BGCParams:
     k_scale: float = 1.0
 
 
-def pheno(v, a10, td, eps, pp: PhenoParams, stresstu, stresstl):
+def daylength_hours(lat_rad, doy):
+    """Astronomical day length (h): declination 0.40910518 sin(2 pi (doy - 81) / 365) and the
+    sunrise hour angle, clipped for polar day and night (state/seasonal.py:53-58)."""
+    decl = 0.40910518 * np.sin(2.0 * np.pi * (doy - 81.0) / 365.0)
+    x = np.clip(-np.tan(lat_rad) * np.tan(decl), -1.0, 1.0)
+    return 24.0 / np.pi * np.arccos(x)
+
+
+def seasonal_update(sea, td, lat_rad, jday, pp: PhenoParams, acclim_tau):
+    """Advance the seasonal states once a day from the day's mean temperature
+    (loops_gswp3.py:418-440): the acclimation state S += (T - S) / tau and the senescence sums,
+    which accumulate max(Tb - T, 0) while day length is decreasing and below Pstart and reset
+    otherwise.  ``sea``: dict acclim_x, ssen_u, ssen_l.  Returns the new states and the
+    senescence triggers (None where the canopy's photoperiod switch is off)."""
+    tc = td - TMELT
+    dl = daylength_hours(lat_rad, jday)
+    dec = dl < daylength_hours(lat_rad, jday - 1)
+    new = dict(acclim_x=sea["acclim_x"] + (tc - sea["acclim_x"]) / acclim_tau,
+               ssen_u=np.where(dec & (dl < pp.sen_pstart_u),
+                               sea["ssen_u"] + np.maximum(pp.sen_tb_u - tc, 0.0), 0.0),
+               ssen_l=np.where(dec & (dl < pp.sen_pstart_l),
+                               sea["ssen_l"] + np.maximum(pp.sen_tb_l - tc, 0.0), 0.0))
+    on_u = new["ssen_u"] >= pp.sen_ycrit_u if pp.sen_photoperiod_u else None
+    on_l = new["ssen_l"] >= pp.sen_ycrit_l if pp.sen_photoperiod_l else None
+    return new, on_u, on_l
+
+
+def pheno(v, a10, td, eps, pp: PhenoParams, stresstu, stresstl, sen_on_u=None, sen_on_l=None):
     """Leaf phenology (physics/phenology.py:94-287).  v: dict of vegetation fields (tempu,
     templ, agddu, agddl, dropu, dropls, dropl3, dropl4 (n,), plai (n,12), tc, frac (n,12),
     lai (n,2), ztop, zbot (n,2), fu, fl); a10: 10-day means (a10td, a10ancub, a10ancls,
-    a10ancl3, a10ancl4).  Returns the updated fields."""
+    a10ancl3, a10ancl4); ``sen_on_u`` / ``sen_on_l``: the photoperiod senescence triggers,
+    which force the cold branch and the degree-day reset (:148-150, :170-172).  Returns the
+    updated fields."""
     ddfac = 1.0 / pp.ddays
     a10td = a10["a10td"]
     # cold phenology, upper (:140-158) and lower (:160-178) canopy
     tthr_u = np.maximum(np.maximum(0.0 + TMELT, v["tc"] + pp.tthresh_u_offset + TMELT),
                         pp.tthresh_u_abs + TMELT)
     gthr_u = pp.gthresh_u_c + TMELT
-    agddu = np.where(a10td < gthr_u, 0.0, v["agddu"] + td - gthr_u)
-    tempu = np.where(a10td < tthr_u, np.maximum(0.0, v["tempu"] - ddfac),
+    cold_u, greset_u = a10td < tthr_u, a10td < gthr_u
+    if sen_on_u is not None:
+        cold_u, greset_u = cold_u | sen_on_u, greset_u | sen_on_u
+    agddu = np.where(greset_u, 0.0, v["agddu"] + td - gthr_u)
+    tempu = np.where(cold_u, np.maximum(0.0, v["tempu"] - ddfac),
                      np.minimum(1.0, np.maximum(0.0, agddu - pp.gdd_thresh_u) / pp.gdd_ramp_u))
     tthr_l = pp.tthresh_l_c + TMELT
     gthr_l = pp.gthresh_l_c + TMELT
-    agddl = np.where(a10td < gthr_l, 0.0, v["agddl"] + td - gthr_l)
-    templ = np.where(a10td < tthr_l, np.maximum(0.0, v["templ"] - ddfac),
+    cold_l, greset_l = a10td < tthr_l, a10td < gthr_l
+    if sen_on_l is not None:
+        cold_l, greset_l = cold_l | sen_on_l, greset_l | sen_on_l
+    agddl = np.where(greset_l, 0.0, v["agddl"] + td - gthr_l)
+    templ = np.where(cold_l, np.maximum(0.0, v["templ"] - ddfac),
                      np.minimum(1.0, np.maximum(0.0, agddl - pp.gdd_thresh_l) / pp.gdd_ramp_l))
 
     # drought phenology (:180-203)

@@ -37,7 +37,9 @@ _BROAD, _CONIF = (0, 1, 2, 4, 6), (3, 5, 7)
 def acc_increments(st, vx, p):
     """(NACC, N) increments of one step.  ``st``: dict of state fields after the step
     (reference shapes); ``vx``: dict with cbiow (N, 12), cbior (N, 12), sapfrac, tco2mic;
-    ``p``: namespace with rwood, rroot, rgrowth (12,), froot (6, 2), dtime, ndaypy."""
+    ``p``: namespace with rwood, rroot, rgrowth (12,), froot (6, 2), dtime, ndaypy and, for
+    StomataParams.gpp_split_by_capacity (loops_gswp3.py:197-206), ``vmax`` (N, 12) per-slot
+    Vmax: the NPP of each upper and shrub big leaf is then shared by leaf area x Vmax."""
     dt = p.dtime
     an_u, an_l = st["canopy.an_upper"], st["canopy.an_lower"]
     ag_u, ag_l = st["canopy.ag_upper"], st["canopy.ag_lower"]
@@ -58,14 +60,21 @@ def acc_increments(st, vx, p):
     rg = np.asarray(p.rgrowth)
     inc = np.zeros((NACC,) + fi.shape)
     npp_tot = np.zeros_like(fi)
+    share = frac
+    vmax = getattr(p, "vmax", None)
+    if vmax is not None:
+        from .stomata import capacity_shares
+        share = frac.copy()
+        for idx in (_BROAD, _CONIF, (8, 9)):
+            share[:, list(idx)] = capacity_shares(frac, vmax, idx)
     for j in range(12):
         if j in _BROAD or j in _CONIF:
             an = an_u[:, 1] if j in _CONIF else an_u[:, 0]
-            gpp = an * frac[:, j] * lai_u * fu
+            gpp = an * share[:, j] * lai_u * fu
             resp = (rw[j] * vx["cbiow"][:, j] * vx["sapfrac"] * funca
                     + rr[j] * vx["cbior"][:, j] * funcb)
         elif j in (8, 9):
-            gpp = an_l[:, 0] * frac[:, j] * lai_l * fl * smask
+            gpp = an_l[:, 0] * share[:, j] * lai_l * fl * smask
             resp = rw[j] * vx["cbiow"][:, j] * funca + rr[j] * vx["cbior"][:, j] * funcb
         else:
             an = an_l[:, 2] if j == 10 else an_l[:, 1]

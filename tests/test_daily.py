@@ -54,7 +54,10 @@ def _unpack_day(a):
             for k, w in DY.DAY_FIELDS}
 
 
-def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1):
+def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1, seasonal=None):
+    """One day begin (end=0) or day end (end=1) of the host build.  ``seasonal``: optional
+    dict with the (3, n) state block, lat_rad, jday and a daily.PhenoParams; the block is
+    updated in place."""
     n = len(td)
     dt = np.float64 if prec == "f64" else np.float32
     st = abi.pack(f, n, dtype=dt)
@@ -67,7 +70,12 @@ def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1):
         acc[j] = sums[k] + comp[k]
         acc[DR.NACC + j] = comp[k]
     vx = np.zeros((DR.NVEGX, n), dtype=dt)
-    p = DY.day_params(HSOI, nsoilcall=nsoilcall)
+    p = DY.day_params(HSOI, nsoilcall=nsoilcall,
+                      pheno=seasonal["pheno"] if seasonal else None)
+    if seasonal:
+        p.seasonal = seasonal["block"].ctypes.data
+        p.lat_rad = seasonal["lat_rad"].ctypes.data
+        p.jday = seasonal["jday"]
     ptr = lambda x: x.ctypes.data_as(C.c_void_p)
     lib = C.CDLL(HOSTCHECK)
     flags = abi.PREC_F64 if prec == "f64" else abi.PREC_MIXED
@@ -157,8 +165,47 @@ def test_host_day_end_matches_the_oracle(prec, nsoilcall):
 L_TCO2MIC = 25
 
 
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("jday", [100, 250, 300])
+def test_host_day_begin_with_seasonal_states_matches_the_oracle(prec, jday):
+    """Acclimation and senescence sums advance, and the photoperiod triggers gate the cold
+    branch, as in the oracle (loops_gswp3.py:418-440, phenology.py:148-172)."""
+    n = 3000
+    f, day, td, sums, comp = day_case(n, seed=9)
+    rng = np.random.default_rng(10)
+    lat = np.deg2rad(rng.uniform(-70, 70, n))
+    dt = np.float64 if prec == "f64" else np.float32
+    block = np.stack([rng.uniform(-15, 25, n), rng.uniform(0, 400, n), rng.uniform(0, 400, n)])
+    if prec == "mixed":
+        f, day, td = _round(f, dt), _round(day, dt), np.asarray(td, dt).astype(np.float64)
+        block = block.astype(dt).astype(np.float64)
+    pp = DY.PhenoParams(sen_photoperiod_u=True, sen_photoperiod_l=True, sen_ycrit_u=148.0)
+    blk = np.ascontiguousarray(block.astype(dt))
+    st, da, _, _ = host_day(0, f, day, td, sums, comp, prec,
+                            seasonal=dict(block=blk, lat_rad=lat, jday=jday, pheno=pp))
+    op = OD.PhenoParams(sen_photoperiod_u=True, sen_photoperiod_l=True, sen_ycrit_u=148.0)
+    sea = dict(acclim_x=block[0], ssen_u=block[1], ssen_l=block[2])
+    new, on_u, on_l = OD.seasonal_update(sea, td, lat, jday, op, 10.93)
+    assert on_u.any() or jday < 172                   # days shorten after the solstice
+    v = {k: day[k] for k in ("tempu", "templ", "agddu", "agddl", "dropu", "dropls", "dropl3",
+                             "dropl4", "plai", "tc")}
+    for k in ("frac", "lai", "ztop", "zbot", "fu", "fl"):
+        v[k] = np.asarray(f["vegetation." + k], dtype=np.float64)
+    a10 = {k: day[k] for k in ("a10td", "a10ancub", "a10ancls", "a10ancl3", "a10ancl4")}
+    ref = OD.pheno(v, a10, td, 1.0e-7, op, f["soil.stresstu"], f["soil.stresstl"],
+                   sen_on_u=on_u, sen_on_l=on_l)
+    tol = 1e-13 if prec == "f64" else 1e-5
+    for j, k in enumerate(DY.SEASONAL_FIELDS):
+        err = np.abs(blk[j] - new[k]).max() / max(np.abs(new[k]).max(), 1.0)
+        assert err <= tol, (k, err)
+    # a trigger computed on the other side of ycrit in FP32 flips a whole branch: count them
+    for k in ("tempu", "templ", "agddu", "agddl"):
+        err = np.abs(da[k] - ref[k]) / np.maximum(np.abs(ref[k]).max(), 1e-30)
+        assert (err > tol).sum() <= (0 if prec == "f64" else 3), (k, int((err > tol).sum()))
+
+
 def test_day_params_struct_layout():
-    assert C.sizeof(DY.IbisDayParams) == 75 * 8 + 8
+    assert C.sizeof(DY.IbisDayParams) == 75 * 8 + 8 + 7 * 8 + 4 * 4 + 2 * 8
     p = DY.day_params(HSOI)
     assert p.cnr[4] == 100.0 and p.klm == 0.15 and p.ynb == 0.25 and p.k_scale == 1.0
     assert p.hsoi[6] == 4.0 and p.nsoilcall == 1 and p.ddays == 15.0

@@ -5,6 +5,7 @@ The GPU kernels are checked in ``test_gpu_driver.py``.
 """
 import ctypes as C
 import math
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -187,8 +188,9 @@ def test_host_diurnal_matches_the_oracle():
             np.testing.assert_allclose(got[k], v, rtol=1e-12, atol=1e-300, err_msg=k)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_host_accumulators_match_the_oracle(dtype):
+def test_host_accumulators_match_the_oracle(dtype, split):
     n = 3000
     st, vx, op = _acc_case(n)
     st = {k: np.asarray(v, dtype).astype(np.float64) for k, v in st.items()}
@@ -207,6 +209,12 @@ def test_host_accumulators_match_the_oracle(dtype):
     for k in range(6):
         p.froot[k][0], p.froot[k][1] = float(op.froot[k, 0]), float(op.froot[k, 1])
     p.dtime, p.ndaypy = 3600.0, 365.0
+    if split:                                  # gpp_split_by_capacity over random slot types
+        slot = np.random.default_rng(4).integers(-1, 40, (12, n)).astype(np.int8)
+        table = synth.type_table(np.linspace(15e-6, 70e-6, 12), {3: 90e-6, 17: 5e-6})
+        p.split_by_capacity, p.slot_pitch = 1, n
+        p.slot_type, p.vmax_table = slot.ctypes.data, table.ctypes.data
+        op.vmax = table[np.where(slot.T >= 0, slot.T, 40 + np.arange(12)[None, :])]
     flags = abi.PREC_F64 if dtype == np.float64 else abi.PREC_MIXED
     lib = _lib()
     for _ in range(3):
@@ -216,6 +224,12 @@ def test_host_accumulators_match_the_oracle(dtype):
                                            C.byref(p), C.c_int64(n), C.c_int(flags))
         assert rc == 0
     ref = 3.0 * OD.acc_increments(st, vx, op)
+    if split:                                  # the split moves NPP among the plant types
+        base = dict(vars(op))
+        base.pop("vmax")
+        ref0 = 3.0 * OD.acc_increments(st, vx, SimpleNamespace(**base))
+        assert not np.allclose(ref[7:19], ref0[7:19])
+        np.testing.assert_allclose(ref[19], ref0[19], rtol=1e-9)      # total GPP unchanged
     tot = acc[:DR.NACC].astype(np.float64) - acc[DR.NACC:].astype(np.float64)
     tol = 1e-12 if dtype == np.float64 else 3e-5
     for j, name in enumerate(DR.ACC_NAMES):

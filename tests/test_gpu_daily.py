@@ -46,6 +46,46 @@ def test_day_begin_kernel_is_the_host_build(prec):
     np.testing.assert_array_equal(acc, hacc)
 
 
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_day_begin_with_seasonal_states_matches_the_host_build(prec):
+    """The seasonal states (sin, tan, acos of the day length) on the device against the host
+    build; the senescence triggers they set gate the same cold branches."""
+    n = 4000
+    f, day, td, sums, comp = day_case(n, seed=13)
+    rng = np.random.default_rng(14)
+    lat_deg = rng.uniform(-70, 70, n)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    npdt = np.float64 if prec == "f64" else np.float32
+    block0 = np.stack([rng.uniform(-15, 25, n), rng.uniform(0, 400, n),
+                       rng.uniform(0, 400, n)]).astype(npdt)
+    pp = DY.PhenoParams(sen_photoperiod_u=True, sen_photoperiod_l=True, sen_ycrit_u=148.0)
+    st = IBISState.from_numpy(f, None, HSOI, root_fractions(), device="cuda", dtype=dt)
+    ds = DY.DayState(st, day)
+    w = torch.zeros((len(DR.DAILY_NAMES), n), dtype=dt, device="cuda")
+    w[0] = torch.as_tensor(td, device="cuda").to(dt)
+    acc = DR.Accumulators(st)
+    p = DY.day_params(HSOI, pheno=pp)
+    blk = DY.attach_seasonal(st, p, lat_deg)
+    blk.copy_(torch.as_tensor(block0, device="cuda"))
+    p.jday = 280
+    DY.day_begin(st, ds, w, acc, p)
+    torch.cuda.synchronize()
+    got_blk = blk.cpu().numpy().astype(np.float64)
+    da = ds.to_numpy()
+    hblk = block0.copy()
+    _, hda, _, _ = host_day(0, f, day, td, sums, comp, prec,
+                            seasonal=dict(block=hblk, lat_rad=np.deg2rad(lat_deg), jday=280,
+                                          pheno=pp))
+    tol = 1e-13 if prec == "f64" else 2e-6
+    for j, k in enumerate(DY.SEASONAL_FIELDS):
+        err = np.abs(got_blk[j] - hblk[j]).max() / max(np.abs(hblk[j]).max(), 1.0)
+        assert err <= tol, (k, err)
+    assert np.array_equal(st.seasonal.acclim_x.cpu().numpy(), blk[0, :n].cpu().numpy())
+    for k in ("tempu", "templ", "agddu", "agddl"):
+        flips = (np.abs(da[k] - hda[k]) > tol * max(np.abs(hda[k]).max(), 1.0)).sum()
+        assert flips <= (0 if prec == "f64" else 3), (k, int(flips))
+
+
 @pytest.mark.parametrize("prec,nsoilcall", [("f64", 1), ("f64", 2), ("mixed", 1)])
 def test_day_end_kernel_matches_the_host_build_and_the_oracle(prec, nsoilcall):
     f, day, td, sums, comp = day_case(4000, seed=12)
@@ -110,3 +150,44 @@ def test_three_days_of_the_day_loop():
         err = np.abs(d32[k] - d64[k]).max() / max(scale, 1e-30)
         print(f"\n{k}: mixed vs FP64 after 3 days, max err / field scale {err:.2e}")
         assert err < 1e-3, k
+
+
+def test_release_configuration_day_loop():
+    """Three days of the day loop with the release parameter file's switches: the seasonal
+    states advance at day begin and feed the hourly step's acclimation; the capacity-weighted
+    NPP split in the accumulators; every field finite, and the mixed run within 1e-3 of FP64."""
+    from ibis_amd.params import release_switches
+    n = 2048
+    f, slot, lat = synth.make_cells(n, 22)
+    out = {}
+    for prec, dt in (("f64", torch.float64), ("mixed", torch.float32)):
+        st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda", dtype=dt)
+        params = release_switches(default_sim_params())
+        rng = np.random.default_rng(3)
+        vals = dict(tempu=np.full(n, 0.5), templ=np.full(n, 0.5), agddu=np.full(n, 80.0),
+                    agddl=np.full(n, 120.0), dropu=np.ones(n), dropls=np.ones(n),
+                    dropl3=np.ones(n), dropl4=np.ones(n), plai=rng.uniform(0.5, 5, (n, 12)),
+                    tc=rng.uniform(-20, 15, n), a10td=np.asarray(f["soil.tsoi"])[:, 0],
+                    csoipas=np.full(n, 8.0), totcmic=np.full(n, 0.1))
+        day = DY.DayState(st, vals)
+        acc = DR.Accumulators(st, params=params)
+        w = DY.SyntheticWeather(lat, np.asarray(f["atm.psurf"]), seed=5, jday0=250)
+        p = DY.day_params(synth.HSOI, pheno=DY.PhenoParams(sen_photoperiod_u=True,
+                                                           sen_photoperiod_l=True))
+        blk = DY.attach_seasonal(st, p, lat, acclim_x0=5.0)
+        DY.run_days(st, day, w, acc, get_constants(), params, 3, p)
+        torch.cuda.synchronize()
+        out[prec] = (st.to_numpy(), day.to_numpy(), acc.values(),
+                     blk[:, :n].cpu().numpy().astype(np.float64))
+    s64, d64, a64, b64 = out["f64"]
+    s32, d32, a32, b32 = out["mixed"]
+    for d in (d64, d32, a64, a32):
+        for k, v in d.items():
+            assert np.isfinite(v).all(), k
+    assert not np.allclose(b64[0], 5.0)                              # acclimation moved
+    assert (b64[1] > 0).any()                                        # September: days shorten
+    for name, x, y in (("acclim_x", b64[0], b32[0]), ("aygpptot", a64["aygpptot"], a32["aygpptot"]),
+                       ("agddu", d64["agddu"], d32["agddu"])):
+        err = np.abs(y - x).max() / max(np.abs(x).max(), 1e-30)
+        print(f"\n{name}: mixed vs FP64 after 3 release-configuration days, {err:.2e}")
+        assert err < 1e-3, name

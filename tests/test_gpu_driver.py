@@ -61,16 +61,25 @@ def test_diurnal_kernel_matches_the_oracle():
     np.testing.assert_allclose(total, dw["precip"], rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_accumulate_kernel_matches_the_oracle(dtype):
+def test_accumulate_kernel_matches_the_oracle(dtype, split):
     n = 5000
     stf, vx, op = _acc_case(n)
     npdt = np.float64 if dtype == torch.float64 else np.float32
     stf = {k: np.asarray(v, npdt).astype(np.float64) for k, v in stf.items()}
     vx = {k: np.asarray(v, npdt).astype(np.float64) for k, v in vx.items()}
     slot = np.full((n, 12), -1, np.int8)
+    params = None
+    if split:                                  # gpp_split_by_capacity over random slot types
+        slot = np.random.default_rng(4).integers(-1, 40, (n, 12)).astype(np.int8)
+        params = default_sim_params()
+        params.stomata_params.gpp_split_by_capacity = True
+        params.vmax_type_overrides = {3: 90e-6, 17: 5e-6}
+        table = synth.type_table(params.stomata_params.vmax_pft, params.vmax_type_overrides)
+        op.vmax = table[np.where(slot >= 0, slot, 40 + np.arange(12)[None, :])]
     st = _state(stf, slot, dtype)
-    acc = DR.Accumulators(st, vegx=vx)
+    acc = DR.Accumulators(st, vegx=vx, params=params)
     for _ in range(5):
         acc.add(st)
     torch.cuda.synchronize()

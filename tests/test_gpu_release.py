@@ -1,0 +1,109 @@
+"""The HIP step with the optional StomataParams switches (``ibis_step_ext``) on the GPU:
+against the oracle, against the host build of the same source, and with no switch set
+against the plain step."""
+import numpy as np
+import pytest
+import torch
+
+from ibis_amd import synth
+from ibis_amd.params import default_sim_params, get_constants
+from ibis_amd.state import IBISState
+from ibis_amd.step import timestep_, timestep_body
+from oracle import params as OP
+
+from _helpers import (assert_parity, capscale_params, host_step_ext, oracle_with_band,
+                      release_params, rounded, seasonal, workload)
+
+pytestmark = pytest.mark.gpu
+
+DT = {"f64": torch.float64, "mixed": torch.float32}
+
+
+def _state(f, slot, prec, aux=None):
+    return IBISState.from_numpy(dict(f, **(aux or {})), slot, synth.HSOI, OP.root_fractions(),
+                                device="cuda", dtype=DT[prec])
+
+
+def _params(config, n, prec):
+    sim = release_params(n) if config == "release" else capscale_params()
+    if prec != "f64" and np.ndim(sim.stomata_params.stressfac):
+        sim.stomata_params.stressfac = np.asarray(sim.stomata_params.stressfac,
+                                                  np.float32).astype(np.float64)
+    return sim
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("config", ["release", "capscale"])
+def test_switches_match_the_oracle_and_the_host_build(prec, config):
+    n = 4000
+    print(f"\nrelease gpu: {config} {prec} n={n}", flush=True)
+    f, slot = workload(n, seed=81, hour=13, prec=prec)
+    sim = _params(config, n, prec)
+    aux = rounded(seasonal(n), prec)
+    band, refs, keys, tol = oracle_with_band(dict(f, **aux), slot, prec, params=sim)
+    new = timestep_body(_state(f, slot, prec, aux), get_constants(), sim, 3600.0,
+                        precision=prec)
+    torch.cuda.synchronize()
+    got = new.to_numpy()
+    assert_parity(got, refs, band, keys, prec, f"gpu {config}")
+    # the host build of the same source: away from the delta band and the snow-skin knife
+    # edge (where either branch is right) the FP64 builds agree to libm rounding carried
+    # through the implicit solves (the oracle and the host build differ by up to 1e-9 here)
+    host = host_step_ext(f, slot, prec, sim, aux)
+    calm = ~band & ~np.asarray(refs[0]["_knife_edge"]).any(axis=1)
+    err = np.zeros(n)
+    for k in keys:
+        a, b = np.asarray(got[k]).reshape(n, -1), np.asarray(host[k]).reshape(n, -1)
+        scale = max(float(np.abs(b).max()), 1e-30)
+        err = np.maximum(err, (np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * scale)).max(axis=1))
+    worst = float(err[calm].max(initial=0.0))
+    print(f"gpu vs host build, {config} {prec}: {int(calm.sum())} cells, worst {worst:.2e}")
+    if prec == "f64":
+        assert worst < 1e-7
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_no_switch_is_the_plain_step_bit_for_bit(prec):
+    """ibis_step_ext with every switch off (a per-cell stressfac equal to the scalar takes
+    the ext path) performs the plain step's arithmetic."""
+    n = 3000
+    f, slot = workload(n, seed=82, hour=13, prec=prec)
+    plain = default_sim_params()
+    ext = default_sim_params()
+    ext.stomata_params.stressfac = np.full(n, plain.stomata_params.stressfac)
+    assert ext.stomata_params.switches_on()
+    a = timestep_body(_state(f, slot, prec), get_constants(), plain, 3600.0, precision=prec)
+    b = timestep_body(_state(f, slot, prec), get_constants(), ext, 3600.0, precision=prec)
+    torch.cuda.synchronize()
+    assert torch.equal(a.arena, b.arena)
+
+
+def test_functional_operator_equals_the_in_place_launch():
+    n = 2000
+    f, slot = workload(n, seed=83, hour=12, prec="mixed")
+    sim = release_params(n)
+    aux = rounded(seasonal(n), "mixed")
+    st = _state(f, slot, "mixed", aux)
+    a = timestep_body(st, get_constants(), sim, 3600.0)
+    b = timestep_(st.clone(), get_constants(), sim, 3600.0)
+    torch.cuda.synchronize()
+    assert torch.equal(a.arena, b.arena)
+    assert torch.equal(a.seasonal.acclim_x, st.seasonal.acclim_x)
+
+
+def test_switches_stay_finite_over_a_day():
+    n = 3000
+    f, slot = workload(n, seed=84, hour=0, prec="mixed")
+    sim = release_params(n)
+    aux = rounded(seasonal(n), "mixed")
+    st = _state(f, slot, "mixed", aux)
+    lat = np.asarray(synth.make_cells(n, 84)[2])
+    for h in range(24):
+        fo = synth.forcing(n, 84, h, lat, f["atm.psurf"])
+        for k, v in fo.items():
+            setattr(getattr(st, k.split(".")[0]), k.split(".")[1], v)
+        timestep_(st, get_constants(), sim, 3600.0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(st.arena[:, :n]).all()
+    gpp = st.canopy.ag_upper.double().cpu().numpy()
+    assert (gpp >= 0).all() and gpp.max() > 1e-6

@@ -182,3 +182,59 @@ def test_nsoilcall_two_is_two_days_of_soilbgc():
     np.testing.assert_allclose(out2["a10ancub"], z * a10["a10ancub"] + (1 - z) * sums["a10ancub"] / 86400.0,
                                rtol=1e-15)
     np.testing.assert_allclose(out2["ayprcp"], sums["adrain"] + sums["adsnow"], rtol=1e-15)
+
+
+# ---- seasonal states (state/seasonal.py; tests/test_seasonal_formulations.py) -------------
+
+def test_daylength_is_astronomically_right():
+    eq = OD.daylength_hours(np.zeros(365), np.arange(1, 366))
+    assert np.all(np.abs(eq - 12.0) < 0.05)
+    n45 = OD.daylength_hours(np.deg2rad(45.0), 172)
+    s45 = OD.daylength_hours(np.deg2rad(45.0), 355)
+    assert 15.2 < n45 < 15.6 and 8.4 < s45 < 8.8
+    assert abs(OD.daylength_hours(np.deg2rad(-45.0), 172) - s45) < 0.1
+    assert OD.daylength_hours(np.deg2rad(80.0), 172) == 24.0
+    assert OD.daylength_hours(np.deg2rad(80.0), 355) == 0.0
+
+
+def test_acclimation_recursion_matches_preles():
+    """S = S + (T - S) / tau, f_S = clip((S - S0) / Smax, 0, 1) (PRELES fS_model)."""
+    pp = OD.PhenoParams()
+    sea = dict(acclim_x=np.zeros(1), ssen_u=np.zeros(1), ssen_l=np.zeros(1))
+    for t in np.r_[np.full(40, -15.0), np.full(60, 20.0)]:
+        sea, _, _ = OD.seasonal_update(sea, np.array([t + 273.16]), np.zeros(1), 100, pp, 10.93)
+    assert sea["acclim_x"][0] > 19.0 and (sea["acclim_x"][0] + 3.063) / 17.72 >= 1.0
+    x = 0.0
+    for t in np.full(60, -15.0):
+        x = x + (t - x) / 10.93
+    assert min(max((x + 3.063) / 17.72, 0.0), 1.0) == 0.0
+
+
+def test_senescence_sums_accumulate_only_while_days_shorten_below_pstart():
+    pp = OD.PhenoParams(sen_photoperiod_u=True, sen_ycrit_u=30.0)
+    lat = np.deg2rad(np.array([50.0, 50.0, -50.0]))
+    td = np.full(3, 273.16 + 5.0)                 # 15 degree-days below Tb = 20 C per day
+    sea = dict(acclim_x=np.zeros(3), ssen_u=np.array([0.0, 20.0, 20.0]), ssen_l=np.zeros(3))
+    new, on_u, on_l = OD.seasonal_update(sea, td, lat, 260, pp, 10.93)   # September
+    np.testing.assert_allclose(new["ssen_u"][:2], [15.0, 35.0])
+    assert new["ssen_u"][2] == 0.0                # southern spring: days lengthen, reset
+    assert list(on_u) == [False, True, False] and on_l is None
+    spring, _, _ = OD.seasonal_update(sea, td, lat, 100, pp, 10.93)
+    assert spring["ssen_u"][0] == 0.0 and spring["ssen_u"][1] == 0.0
+
+
+def test_senescence_trigger_forces_leaf_fall_and_holds_degree_days():
+    n = 3
+    v, a10 = veg_case(n, 2)
+    v["tempu"], v["agddu"] = np.ones(n), np.full(n, 900.0)
+    a10["a10td"] = np.full(n, 288.0)
+    td = np.full(n, 288.0)
+    pp = OD.PhenoParams()
+    s = np.full(n, 1.0)
+    a = OD.pheno(v, a10, td, 1e-7, pp, s, s)
+    b = OD.pheno(v, a10, td, 1e-7, pp, s, s, sen_on_u=np.zeros(n, bool), sen_on_l=np.zeros(n, bool))
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    c = OD.pheno(v, a10, td, 1e-7, pp, s, s, sen_on_u=np.array([True, False, True]))
+    np.testing.assert_allclose(c["tempu"], [1.0 - 1.0 / 15.0, a["tempu"][1], 1.0 - 1.0 / 15.0])
+    assert c["agddu"][0] == 0.0 and c["agddu"][2] == 0.0 and c["agddu"][1] > 0

@@ -8,57 +8,10 @@ import copy
 import numpy as np
 import pytest
 
-from _helpers import assert_parity, host_step, host_step_ext, oracle_with_band, workload
+from _helpers import (assert_parity, capscale_params, host_step, host_step_ext, oracle_with_band,
+                      release_params, rounded, seasonal, workload)
 from ibis_amd import abi
 from ibis_amd.params import default_sim_params
-
-
-def release_params(n, seed=5):
-    """The switches of python-port/configs/params/params_31s_ch4h.yaml:55-90, :299 with
-    per-type values drawn within the file's ranges and a per-cell stressfac."""
-    rng = np.random.default_rng(seed)
-    sim = default_sim_params()
-    sp = sim.stomata_params
-    sp.sunshade_u = sp.sunshade_l = True
-    sp.vmax_mix = sp.coefm_mix = True
-    sp.acclim_on = sp.acclim_per_pft = True
-    sp.gpp_split_by_capacity = True
-    sp.cap_kn = 0.15
-    sp.coefm_pft = rng.uniform(3.0, 11.0, 12)
-    sp.acclim_s0_pft = np.full(12, -3.063)
-    sp.acclim_smax_pft = np.full(12, 17.72)
-    sp.acclim_s0_pft[[3, 5]] = (-8.594, -6.605)
-    sp.acclim_smax_pft[[3, 5]] = (24.2, 29.76)
-    sp.stressfac = rng.uniform(-9.0, -3.0, n)
-    sim.h2o_consistent_weights = True
-    for t in range(0, 40, 3):
-        sim.vmax_type_overrides[t] = float(rng.uniform(1e-5, 5e-5))
-        sim.coefm_type_overrides[t] = float(rng.uniform(4.0, 16.0))
-    sim.acclim_s0_type_overrides[6] = -1.454
-    sim.acclim_smax_type_overrides[6] = 28.02
-    return sim
-
-
-def capscale_params():
-    sim = default_sim_params()
-    sp = sim.stomata_params
-    sp.cap_scale_u = sp.cap_scale_l = True
-    sp.scale_leaf_only = True
-    sp.acclim_on = sp.acclim_light = sp.frost_on = True
-    sp.cap_kn = 0.2
-    return sim
-
-
-def seasonal(n, seed=7):
-    rng = np.random.default_rng(seed)
-    return {"seasonal.acclim_x": rng.uniform(-15.0, 25.0, n),
-            "atm.td": 273.16 + rng.uniform(-12.0, 12.0, n)}
-
-
-def _round(aux, prec):
-    if prec == "f64":
-        return aux
-    return {k: np.asarray(v, np.float32).astype(np.float64) for k, v in aux.items()}
 
 
 def test_no_switch_is_the_plain_step_bit_for_bit():
@@ -80,7 +33,7 @@ def test_switches_match_the_oracle(prec, config):
     if prec != "f64" and np.ndim(sim.stomata_params.stressfac):
         sim.stomata_params.stressfac = np.asarray(sim.stomata_params.stressfac,
                                                   np.float32).astype(np.float64)
-    aux = _round(seasonal(n), prec)
+    aux = rounded(seasonal(n), prec)
     fo = dict(f, **aux)
     band, refs, keys, tol = oracle_with_band(fo, slot, prec, params=sim)
     got = host_step_ext(f, slot, prec, sim, aux)
