@@ -1,0 +1,58 @@
+"""The reference's parameter files read into the hourly step's parameter objects
+(``ibis_amd.params.load_params_yaml``, after python-port/ibis/io/physics_params.py:129-247)."""
+import os
+
+import numpy as np
+import pytest
+
+from ibis_amd import abi, params as PP
+
+REF = "/root/reference/python-port/configs/params"
+
+
+def test_defaults_of_an_empty_file_are_the_f77_path():
+    sim = PP.sim_params_from_dict({})
+    sp = sim.stomata_params
+    assert not sp.switches_on() and not sim.h2o_consistent_weights
+    np.testing.assert_array_equal(sp.vmax_pft, PP.DEFAULT_VMAX)
+    np.testing.assert_array_equal(sp.coefm_pft, [sp.coefm[g] for g in PP.PFT_SLOPE_GROUP])
+    assert abi.switch_bits(sp) == 0
+
+
+def test_sections_and_types_are_read():
+    d = {"stomata": {"alpha3": 0.07, "sunshade_u": True, "acclim_on": True,
+                     "acclim_pfts": [3, 5], "ball_berry": {"conifer": {"coefm": 5.5}},
+                     "vmax": {"c3_grass": 3e-5}, "coefm_pft": {"trop_bl_ever": 4.0},
+                     "acclim_s0_pft": {"boreal_conifer_evergreen": -6.0}},
+         "soil": {"h2o_consistent_weights": True, "bperm": 0.05},
+         "canopy": {"chs": 2.0e5, "dleaf": [0.2, 0.1]},
+         "types": {"T01": {"vmax": 4.4e-5, "coefm": 7.0}, "G08": {"acclim_smax": 25.0}}}
+    sim = PP.sim_params_from_dict(d)
+    sp = sim.stomata_params
+    assert sp.alpha3 == 0.07 and sp.sunshade_u and sp.acclim_on and sp.acclim_pfts == (3, 5)
+    assert sp.coefm[1] == 5.5 and sp.vmax_pft[11] == 3e-5
+    assert sp.coefm_pft[0] == 4.0 and sp.coefm_pft[3] == 5.5          # group default follows
+    assert sp.acclim_s0_pft[5] == -6.0 and sp.acclim_s0_pft[0] == -3.063
+    assert sim.h2o_consistent_weights and sim.bperm == 0.05 and sim.canopy.chs == 2.0e5
+    assert sim.canopy.dleaf == (0.2, 0.1)
+    assert sim.vmax_type_overrides == {0: 4.4e-5} and sim.coefm_type_overrides == {0: 7.0}
+    assert sim.acclim_smax_type_overrides == {PP.TYPE_IDS.index("G08"): 25.0}
+    assert abi.switch_bits(sp) == abi.SW_SUNSHADE_U | abi.SW_ACCLIM
+    with pytest.raises(KeyError):
+        PP.sim_params_from_dict({"types": {"X99": {"vmax": 1.0}}})
+    with pytest.raises(KeyError):
+        PP.sim_params_from_dict({"stomata": {"coefm_pft": {"oak": 1.0}}})
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="the reference's parameter files are absent")
+def test_release_file_switches_match_release_switches():
+    sim = PP.load_params_yaml(os.path.join(REF, "params_31s_ch4h.yaml"))
+    sp = sim.stomata_params
+    ref = PP.release_switches(PP.default_sim_params())
+    for k in PP.SWITCH_NAMES + ("gpp_split_by_capacity",):
+        assert getattr(sp, k) == getattr(ref.stomata_params, k), k
+    assert sim.h2o_consistent_weights == ref.h2o_consistent_weights
+    assert sp.acclim_pfts == (3, 5, 8) and sp.cap_kn == 0.15
+    assert len(sim.vmax_type_overrides) > 30 and len(sim.coefm_type_overrides) > 30
+    d = PP.load_params_yaml(os.path.join(REF, "default.yaml"))
+    assert not d.stomata_params.switches_on()
