@@ -7,14 +7,16 @@ soil solves), synthetic seeded cells and forcing with the benchmark distribution
 after ``--warmup`` untimed hours.  The forcing of every hour is generated before the timed
 region and kept in HBM; each timed hour copies its 12 forcing planes into the state (as the
 reference driver writes ``state.atm`` before each step, ``loops_gswp3.py:142-152``) and
-launches the step.
+launches the step.  By default the timed hours are captured once into a HIP graph and the
+graph is replayed (``--launch eager`` enqueues each hour from the host instead).
 
 Reported beside the throughput:
 
 * ``roofline``: the step's achieved algorithmic bandwidth -- 1,448 B per cell-hour (179 fp32
   words read, 180 written, 12 int8 slot ids) x cells / mean duration of one ``ibis_step``
   call (its surface, canopy and ground kernels) measured with HIP events on the launch
-  stream -- against MI355X's 8 TB/s.  ``traffic`` is the HBM bytes one call moves, from the
+  stream (with a graph, around the replay, so the hour's forcing copy is included) --
+  against MI355X's 8 TB/s.  ``traffic`` is the HBM bytes one call moves, from the
   calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed in ``profiles/`` for this
   workload (null when no summary matches the run);
 * ``cpu_baseline``: the oracle (NumPy FP64 restatement of the reference step, one core) on a
@@ -59,6 +61,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-hours", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="graph", choices=["eager", "graph"],
+                    help="graph: the timed hours (forcing copy and step of each) captured once "
+                         "into a HIP graph and replayed; eager: each hour enqueued from the "
+                         "host, with HIP events around each step")
     ap.add_argument("--stomata", default="f77", choices=["f77", "release"],
                     help="f77: every optional StomataParams switch off (the default parameter "
                          "file); release: the switches of the release parameter file, a "
@@ -92,6 +98,7 @@ def main():
     st = IBISState.from_numpy(fields, slot, synth.HSOI, root_fractions(), device=dev,
                               dtype=dtype, pitch=pitch)
     cparams = S.ibis_params(st, consts, params)
+    cext = S.ext_inputs(st, params) if S.uses_ext(params) else None
 
     atm0 = L.PLANE_OFFSET["atm.ta"]
     natm = sum(f.width for f in L.fields_of("atm"))
@@ -110,14 +117,14 @@ def main():
     atm_view = st.arena[atm0:atm0 + natm]
     stream = torch.cuda.current_stream(dev)
 
-    def hour(h, ev=None):
+    def hour(h, ev=None, s=stream):
         atm_view.copy_(forc[h])
         if ev is not None:
-            ev[0].record(stream)
-        S.timestep_(st, consts, params, 3600.0, precision=args.precision, stream=stream,
-                    cparams=cparams)
+            ev[0].record(s)
+        S.timestep_(st, consts, params, 3600.0, precision=args.precision, stream=s,
+                    cparams=cparams, cext=cext)
         if ev is not None:
-            ev[1].record(stream)
+            ev[1].record(s)
 
     if args.copy_probe_mib > 0:
         words = args.copy_probe_mib * (1 << 20) // 4
@@ -127,17 +134,41 @@ def main():
         torch.neg(src, out=dst)          # elementwise kernel: reads and writes the size once
         torch.cuda.synchronize()
         del src, dst
-    for h in range(args.warmup):
-        hour(h)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        hour(args.warmup + k, evs[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = [a.elapsed_time(b) for a, b in evs]
+    if args.launch == "graph":
+        # the timed hours as one HIP graph on a capture stream of default priority; the
+        # warm-up hours run on that stream first, which also allocates its workspace
+        cs = torch.cuda.Stream(dev)
+        with torch.cuda.stream(cs):
+            for h in range(args.warmup):
+                hour(h, s=cs)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cs):
+            for k in range(args.steps):
+                hour(args.warmup + k, s=cs)
+        torch.cuda.synchronize()
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        t0 = time.perf_counter()
+        with torch.cuda.stream(cs):
+            ev[0].record(cs)
+            graph.replay()
+            ev[1].record(cs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # per hour: the forcing copy and the three kernels, from events around the replay
+        kms = [ev[0].elapsed_time(ev[1]) / args.steps]
+    else:
+        for h in range(args.warmup):
+            hour(h)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            hour(args.warmup + k, evs[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kms = [a.elapsed_time(b) for a, b in evs]
     kmean = float(np.mean(kms)) * 1e-3
     out = st.to_numpy(keys=("soil.tsoi", "canopy.tu"))
     finite = float(np.mean(np.isfinite(out["soil.tsoi"]).all(1) & np.isfinite(out["canopy.tu"])))
@@ -154,7 +185,7 @@ def main():
         "config": {"workload": f"plain hourly step, {n:,} cells x {args.steps} h, one launch per hour",
                    "cells": n, "hours": args.steps, "precision": args.precision,
                    "soil": "4 m soil + 8 m bedrock", "plant_type_slots": 12,
-                   "stomata": args.stomata},
+                   "stomata": args.stomata, "launch": args.launch},
         "kernel_ms_mean": kmean * 1e3, "kernel_ms_min": float(np.min(kms)),
         "finite_fraction": finite,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
@@ -163,7 +194,9 @@ def main():
                                  if args.stomata == "f77" else None),
                      "bytes_per_cell_hour": per_cell,
                      "scope": ("one ibis_step call" if args.stomata == "f77" else
-                               "one ibis_step_ext call") + ": surface + canopy + ground kernels"},
+                               "one ibis_step_ext call") + ": surface + canopy + ground kernels"
+                              + ("" if args.launch == "eager" else
+                                 ", timed per hour over the graph replay with the forcing copy")},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, args.cpu_hours, args.seed, args.stomata)

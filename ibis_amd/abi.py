@@ -25,7 +25,8 @@ TYPE_ROWS = NTYPE + L.NPFT
 SW_VMAX_MIX, SW_COEFM_MIX, SW_SUNSHADE_U, SW_SUNSHADE_L = 0x001, 0x002, 0x004, 0x008
 SW_CAP_SCALE_U, SW_CAP_SCALE_L, SW_SCALE_LEAF_ONLY = 0x010, 0x020, 0x040
 SW_ACCLIM, SW_ACCLIM_PER_PFT, SW_ACCLIM_LIGHT, SW_FROST = 0x080, 0x100, 0x200, 0x400
-SW_ALL = 0x7ff
+SW_CROP = 0x800
+SW_ALL = 0xfff
 #: StomataParams switch name -> bit (acclim_on maps to SW_ACCLIM)
 SWITCHES = {"vmax_mix": SW_VMAX_MIX, "coefm_mix": SW_COEFM_MIX, "sunshade_u": SW_SUNSHADE_U,
             "sunshade_l": SW_SUNSHADE_L, "cap_scale_u": SW_CAP_SCALE_U,
@@ -73,7 +74,8 @@ class IbisStomataExt(C.Structure):
     _fields_ = [("switches", C.c_int32), ("acclim_mask", C.c_int32), ("cap_kn", _D),
                 ("acclim_s0", _D), ("acclim_smax", _D), ("frost_t0", _D), ("frost_t1", _D),
                 ("type_tables", C.c_void_p), ("acclim_x", C.c_void_p), ("td", C.c_void_p),
-                ("stressfac", C.c_void_p)]
+                ("stressfac", C.c_void_p), ("cropfrac", C.c_void_p), ("crop_c4_share", _D),
+                ("crop_vmax_scale_c3", _D), ("crop_vmax_scale_c4", _D)]
 
 
 def switch_bits(sp) -> int:
@@ -81,8 +83,10 @@ def switch_bits(sp) -> int:
     return sum(bit for name, bit in SWITCHES.items() if bool(getattr(sp, name, False)))
 
 
-def stomata_ext(sp, type_tables=0, acclim_x=0, td=0, stressfac=0) -> IbisStomataExt:
-    """IbisStomataExt from a StomataParams-like object and device pointers."""
+def stomata_ext(sp, type_tables=0, acclim_x=0, td=0, stressfac=0, cropfrac=0,
+                crop=None) -> IbisStomataExt:
+    """IbisStomataExt from a StomataParams-like object, device pointers and, with a cropland
+    share, the CropParams-like ``crop``."""
     x = IbisStomataExt()
     x.switches = switch_bits(sp)
     x.acclim_mask = sum(1 << int(j) for j in getattr(sp, "acclim_pfts", ()))
@@ -92,6 +96,14 @@ def stomata_ext(sp, type_tables=0, acclim_x=0, td=0, stressfac=0) -> IbisStomata
     x.frost_t0 = float(getattr(sp, "frost_t0", -5.0))
     x.frost_t1 = float(getattr(sp, "frost_t1", 5.0))
     x.type_tables, x.acclim_x, x.td, x.stressfac = type_tables, acclim_x, td, stressfac
+    x.crop_c4_share, x.crop_vmax_scale_c3, x.crop_vmax_scale_c4 = -1.0, 1.0, 1.0
+    if cropfrac:
+        x.switches |= SW_CROP
+        x.cropfrac = cropfrac
+        if crop is not None:
+            x.crop_c4_share = float(crop.c4_share)
+            x.crop_vmax_scale_c3 = float(crop.vmax_scale_c3)
+            x.crop_vmax_scale_c4 = float(crop.vmax_scale_c4)
     return x
 
 

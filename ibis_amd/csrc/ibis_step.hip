@@ -146,6 +146,8 @@ int ibis_step_ext(const IbisArena* in, const IbisArena* out, void* workspace,
     return fail(IBIS_E_ARG, "ibis_step_ext: IBIS_SW_ACCLIM needs the acclimation state");
   if (acclim && (ext->switches & IBIS_SW_FROST) && !ext->td)
     return fail(IBIS_E_ARG, "ibis_step_ext: IBIS_SW_FROST needs the daily mean temperature");
+  if ((ext->switches & IBIS_SW_CROP) && !ext->cropfrac)
+    return fail(IBIS_E_ARG, "ibis_step_ext: IBIS_SW_CROP needs the cropland share");
   StepConst K;
   const int rc = check_step("ibis_step_ext", in, out, workspace, workspace_bytes, slot_type,
                             slot_pitch, params, dtime, nytimes, ncell, flags, K);

@@ -45,17 +45,22 @@ struct StepExt {
   const void* acclim_x;
   const void* td;
   const void* stressfac;
+  const void* cropfrac;
+  double crop_c4_share, crop_s3, crop_s4;
   IBIS_HD IBIS_INLINE bool on(int bits) const { return (sw & bits) != 0; }
 };
 
-// The launch's StepExt from the C ABI's IbisStomataExt.  acclim_light, acclim_per_pft and the
-// frost ramp act only through the acclimation factor (loops.py:291-310): without
-// IBIS_SW_ACCLIM they are off.
+// The launch's StepExt from the C ABI's IbisStomataExt.  acclim_per_pft and the frost ramp act
+// only through the acclimation factor, acclim_light through any Vmax factor (loops.py:291-310,
+// stomata.py:832-837): without the factors they are off.
 inline StepExt step_ext_from(const IbisStomataExt& x) {
   StepExt E;
   const bool acclim = (x.switches & IBIS_SW_ACCLIM) != 0;
-  E.sw = acclim ? x.switches
-                : (x.switches & ~(IBIS_SW_ACCLIM_PER_PFT | IBIS_SW_ACCLIM_LIGHT | IBIS_SW_FROST));
+  const bool crop = (x.switches & IBIS_SW_CROP) != 0;
+  int off = 0;
+  if (!acclim) off |= IBIS_SW_ACCLIM_PER_PFT | IBIS_SW_FROST;
+  if (!acclim && !crop) off |= IBIS_SW_ACCLIM_LIGHT;
+  E.sw = x.switches & ~off;
   E.acclim_mask = x.acclim_mask;
   E.cap_kn = x.cap_kn;
   E.acclim_s0 = x.acclim_s0;
@@ -65,6 +70,10 @@ inline StepExt step_ext_from(const IbisStomataExt& x) {
   E.acclim_x = x.acclim_x;
   E.td = x.td;
   E.stressfac = x.stressfac;
+  E.cropfrac = x.cropfrac;
+  E.crop_c4_share = x.crop_c4_share;
+  E.crop_s3 = x.crop_vmax_scale_c3;
+  E.crop_s4 = x.crop_vmax_scale_c4;
   return E;
 }
 
@@ -370,7 +379,12 @@ IBIS_HD void big_leaves(StomExt<R>& X, const Cell<S>& c, const StepExt& E,
     frac[j] = LD(IBIS_P_VEGETATION_FRAC + j);
     vp[j] = vrow[j];
   }
-  const bool acclim = E.on(IBIS_SW_ACCLIM);
+  // Vmax factor per plant type (loops.py:291-310, crop_vfac :217-231): acclimation x frost on
+  // the acclimating types, then the cropland factor on the grasses
+  const bool acclim = E.on(IBIS_SW_ACCLIM), crop = E.on(IBIS_SW_CROP);
+  R vfac[IBIS_NPFT];
+#pragma unroll
+  for (int j = 0; j < IBIS_NPFT; ++j) vfac[j] = R(1);
   if (acclim) {
     const R ax = R(((const S*)E.acclim_x)[c.i]);
     R ff = R(1);
@@ -383,9 +397,21 @@ IBIS_HD void big_leaves(StomExt<R>& X, const Cell<S>& c, const StepExt& E,
       if (E.acclim_mask & (1 << j)) {
         const R s0 = E.on(IBIS_SW_ACCLIM_PER_PFT) ? xrow[1][j] : R(E.acclim_s0);
         const R smax = E.on(IBIS_SW_ACCLIM_PER_PFT) ? xrow[2][j] : R(E.acclim_smax);
-        vp[j] = vrow[j] * (clip((ax - s0) / smax, R(0), R(1)) * ff);
+        vfac[j] = clip((ax - s0) / smax, R(0), R(1)) * ff;
       }
     }
+  }
+  if (crop) {
+    const R cf = R(((const S*)E.cropfrac)[c.i]);
+    const R q = R(E.crop_c4_share), s3 = R(E.crop_s3), s4 = R(E.crop_s4);
+    const R qc = clip(q, R(0), R(1));
+    const R smix = (R(1) - qc) * s3 + qc * s4;
+    vfac[10] = vfac[10] * (R(1) + cf * (q < R(0) ? s4 - R(1) : smix - R(1)));
+    vfac[11] = vfac[11] * (R(1) + cf * (q < R(0) ? s3 - R(1) : smix - R(1)));
+  }
+  if (acclim || crop) {
+#pragma unroll
+    for (int j = 0; j < IBIS_NPFT; ++j) vp[j] = vrow[j] * vfac[j];
   }
   const R ex0 = LD(IBIS_P_VEGETATION_EXIST + 0), ex2 = LD(IBIS_P_VEGETATION_EXIST + 2);
   const R ex3 = LD(IBIS_P_VEGETATION_EXIST + 3);
@@ -404,7 +430,7 @@ IBIS_HD void big_leaves(StomExt<R>& X, const Cell<S>& c, const StepExt& E,
     out[4] = v[10];
   };
   select(vp, X.vnom);
-  if (acclim && E.on(IBIS_SW_ACCLIM_LIGHT)) {
+  if ((acclim || crop) && E.on(IBIS_SW_ACCLIM_LIGHT)) {
     R raw[5];
     select(vrow, raw);
 #pragma unroll

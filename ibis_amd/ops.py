@@ -6,7 +6,7 @@
                          ncell, flags) -> ()           # in place
 
     torch.ops.ibis.step_ext(arena, workspace, slot_type, tables, params, ext, acclim_x, td,
-                            stressfac, dtime, nytimes, ncell, flags) -> Tensor
+                            stressfac, cropfrac, dtime, nytimes, ncell, flags) -> Tensor
 
 ``arena`` is the plane-major state (``(IBIS_NPLANE, pitch)``, float32 or float64, on the
 GPU), ``workspace`` a uint8 device tensor of ``ibis_step_workspace_bytes`` bytes,
@@ -15,7 +15,7 @@ float64 Vmax rows on the device, ``params`` the ``IbisParams`` block as a CPU ui
 (``ibis_amd.abi.params_bytes``), ``flags`` the precision and bedrock bits of ``ibis_step``.
 ``step_ext`` is ``ibis_step_ext``: ``tables`` the IBIS_NTABLE x 52 per-slot tables on the
 device, ``ext`` the switch block of ``ext_block`` (CPU float64), ``acclim_x`` / ``td`` /
-``stressfac`` optional (pitch,) device planes of the arena's dtype.
+``stressfac`` / ``cropfrac`` optional (pitch,) device planes of the arena's dtype.
 Both run on the current stream of the arena's device and synchronise nothing.  They are
 registered with ``torch.library`` (the Python form of TORCH_LIBRARY), with fake
 implementations for tracing.  Replaces ``timestep_body``
@@ -88,7 +88,7 @@ def _(arena, workspace, slot_type, type_table, params, dtime, nytimes, ncell, fl
 
 
 _EXT_SCALARS = ("switches", "acclim_mask", "cap_kn", "acclim_s0", "acclim_smax", "frost_t0",
-                "frost_t1")
+                "frost_t1", "crop_c4_share", "crop_vmax_scale_c3", "crop_vmax_scale_c4")
 
 
 def ext_block(x: "abi.IbisStomataExt") -> torch.Tensor:
@@ -97,7 +97,7 @@ def ext_block(x: "abi.IbisStomataExt") -> torch.Tensor:
 
 
 def _run_ext(arena_in, arena_out, workspace, slot_type, tables, params, ext, acclim_x, td,
-             stressfac, dtime, nytimes, ncell, flags):
+             stressfac, cropfrac, dtime, nytimes, ncell, flags):
     if arena_in.device.type != "cuda":
         raise RuntimeError("ibis::step_ext runs on the GPU")
     if params.device.type != "cpu" or params.dtype != torch.uint8 or \
@@ -108,7 +108,8 @@ def _run_ext(arena_in, arena_out, workspace, slot_type, tables, params, ext, acc
     if tables.dtype != torch.float64 or tables.numel() != abi.NTABLE * abi.TYPE_ROWS:
         raise ValueError("tables must hold IBIS_NTABLE x 52 float64 rows")
     pitch = arena_in.shape[1]
-    for name, t in (("acclim_x", acclim_x), ("td", td), ("stressfac", stressfac)):
+    for name, t in (("acclim_x", acclim_x), ("td", td), ("stressfac", stressfac),
+                    ("cropfrac", cropfrac)):
         if t is not None and (t.shape != (pitch,) or t.dtype != arena_in.dtype or
                               t.device != arena_in.device):
             raise ValueError(f"{name} must be a (pitch,) device plane of the arena's dtype")
@@ -119,10 +120,11 @@ def _run_ext(arena_in, arena_out, workspace, slot_type, tables, params, ext, acc
     v = ext.tolist()
     x = abi.IbisStomataExt()
     x.switches, x.acclim_mask = int(v[0]), int(v[1])
-    x.cap_kn, x.acclim_s0, x.acclim_smax, x.frost_t0, x.frost_t1 = v[2:]
+    (x.cap_kn, x.acclim_s0, x.acclim_smax, x.frost_t0, x.frost_t1, x.crop_c4_share,
+     x.crop_vmax_scale_c3, x.crop_vmax_scale_c4) = v[2:]
     ptr = lambda t: t.data_ptr() if t is not None else 0
-    x.type_tables, x.acclim_x, x.td, x.stressfac = (tables.data_ptr(), ptr(acclim_x), ptr(td),
-                                                      ptr(stressfac))
+    x.type_tables, x.acclim_x, x.td, x.stressfac, x.cropfrac = (
+        tables.data_ptr(), ptr(acclim_x), ptr(td), ptr(stressfac), ptr(cropfrac))
     lib = _lib.load()
     ain = abi.IbisArena(arena_in.data_ptr(), pitch)
     aout = abi.IbisArena(arena_out.data_ptr(), pitch)
@@ -140,15 +142,15 @@ def _run_ext(arena_in, arena_out, workspace, slot_type, tables, params, ext, acc
 def step_ext(arena: torch.Tensor, workspace: torch.Tensor, slot_type: Optional[torch.Tensor],
              tables: torch.Tensor, params: torch.Tensor, ext: torch.Tensor,
              acclim_x: Optional[torch.Tensor], td: Optional[torch.Tensor],
-             stressfac: Optional[torch.Tensor], dtime: float, nytimes: float, ncell: int,
-             flags: int) -> torch.Tensor:
+             stressfac: Optional[torch.Tensor], cropfrac: Optional[torch.Tensor], dtime: float,
+             nytimes: float, ncell: int, flags: int) -> torch.Tensor:
     out = torch.empty_like(arena)
     _run_ext(arena, out, workspace, slot_type, tables, params, ext, acclim_x, td, stressfac,
-             dtime, nytimes, ncell, flags | abi.FLAG_COPY_IN)
+             cropfrac, dtime, nytimes, ncell, flags | abi.FLAG_COPY_IN)
     return out
 
 
 @step_ext.register_fake
-def _(arena, workspace, slot_type, tables, params, ext, acclim_x, td, stressfac, dtime, nytimes,
-      ncell, flags):
+def _(arena, workspace, slot_type, tables, params, ext, acclim_x, td, stressfac, cropfrac, dtime,
+      nytimes, ncell, flags):
     return torch.empty_like(arena)

@@ -169,6 +169,17 @@ class FireParams:
 
 
 @dataclass
+class CropParams:
+    """Cropland as managed grass (python-port/ibis/io/parameters.py:262-275): the grasses on a
+    cell's cropland share photosynthesise with Vmax x vmax_scale_c3 (PFT 11) / _c4 (PFT 10);
+    c4_share in 0..1 mixes the two scales on both grasses, -1 keeps them apart."""
+    harvest_index: float = 0.45
+    vmax_scale_c3: float = 1.0
+    vmax_scale_c4: float = 1.0
+    c4_share: float = -1.0
+
+
+@dataclass
 class SimParams:
     veg_rad: VegRadParams = field(default_factory=VegRadParams)
     canopy: CanopyParams = field(default_factory=CanopyParams)
@@ -185,6 +196,9 @@ class SimParams:
     #: per-type overrides {type index 0..39: value} of Vmax, the stomatal slope and the
     #: acclimation constants, IBIS 3.0 zone types (python-port/ibis/io/pft_types.py:170-212)
     vmax_type_overrides: dict = field(default_factory=dict)
+    #: cropland share per cell (SimParams.cropfrac, None: no cropland) and its parameters
+    cropfrac: np.ndarray | None = None
+    crop: CropParams = field(default_factory=CropParams)
     coefm_type_overrides: dict = field(default_factory=dict)
     acclim_s0_type_overrides: dict = field(default_factory=dict)
     acclim_smax_type_overrides: dict = field(default_factory=dict)
@@ -311,6 +325,11 @@ def sim_params_from_dict(d: dict, bedrock: bool = True) -> SimParams:
         if k in so:
             setattr(sim, k, float(so[k]))
     sim.h2o_consistent_weights = bool(so.get("h2o_consistent_weights", False))
+    cr = d.get("crop", {}) or {}
+    for k, v in cr.items():
+        if not hasattr(sim.crop, k):
+            raise KeyError(f"unknown crop parameter {k!r}")
+        setattr(sim.crop, k, float(v))
     fp = d.get("fire", {}) or {}
     for k in ("d_crit", "vpd_ref", "d_crit_vpd"):
         if k in fp:

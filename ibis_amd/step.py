@@ -94,8 +94,15 @@ def _cell_plane(state: IBISState, values, what: str) -> torch.Tensor:
     return t
 
 
+def uses_ext(params: SimParams) -> bool:
+    """Whether the step needs ibis_step_ext: an optional StomataParams switch, a per-cell
+    stressfac or a cropland share."""
+    return (params.stomata_params.switches_on()
+            or getattr(params, "cropfrac", None) is not None)
+
+
 def ext_inputs(state: IBISState, params: SimParams):
-    """(switch block, tables, acclim_x, td, stressfac) of ibis_step_ext for ``state``.
+    """(switch block, tables, acclim_x, td, stressfac, cropfrac) of ibis_step_ext for ``state``.
 
     The acclimation factor needs the seasonal state (timestepping/loops.py:294: without
     ``state.seasonal`` the F77 Vmax is kept); the frost ramp needs ``atm.td``."""
@@ -110,7 +117,14 @@ def ext_inputs(state: IBISState, params: SimParams):
     sf = None
     if np.ndim(sp.stressfac) != 0:
         sf = _cell_plane(state, sp.stressfac, "stomata_params.stressfac")
-    return x, type_tables(params, state.device), acclim_x, td, sf
+    cf = None
+    if getattr(params, "cropfrac", None) is not None:
+        cf = _cell_plane(state, params.cropfrac, "cropfrac")
+        x.switches |= abi.SW_CROP
+        c = params.crop
+        x.crop_c4_share, x.crop_vmax_scale_c3, x.crop_vmax_scale_c4 = (
+            float(c.c4_share), float(c.vmax_scale_c3), float(c.vmax_scale_c4))
+    return x, type_tables(params, state.device), acclim_x, td, sf, cf
 
 
 def ibis_params(state: IBISState, consts: PhysicalConstants, params: SimParams,
@@ -154,8 +168,10 @@ def _check(state_in: IBISState, params):
 
 
 def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: float,
-           precision=None, copy_in=False, stream=None, cparams=None):
-    """Enqueue one hour on ``stream`` (default: torch's current stream of the device)."""
+           precision=None, copy_in=False, stream=None, cparams=None, cext=None):
+    """Enqueue one hour on ``stream`` (default: torch's current stream of the device).
+    ``cparams`` / ``cext``: the IbisParams block and the ``ext_inputs`` of an earlier hour
+    with the same parameters, reused to keep host work out of a driver's hour loop."""
     lib = _lib.load()
     _check(state_in, params)
     code = _precision(state_in, precision)
@@ -172,11 +188,11 @@ def launch(state_in: IBISState, arena_out: torch.Tensor, consts, params, dtime: 
     slot = state_in.slot_type
     ws = workspace(state_in, code, stream)
     slot_ptr = C.c_void_p(slot.data_ptr() if slot is not None else 0)
-    if params.stomata_params.switches_on():
-        x, tables, acclim_x, td, sf = ext_inputs(state_in, params)
+    if uses_ext(params):
+        x, tables, acclim_x, td, sf, cf = cext if cext is not None else ext_inputs(state_in, params)
         ptr = lambda t: t.data_ptr() if t is not None else 0
-        x.type_tables, x.acclim_x, x.td, x.stressfac = (tables.data_ptr(), ptr(acclim_x),
-                                                          ptr(td), ptr(sf))
+        x.type_tables, x.acclim_x, x.td, x.stressfac, x.cropfrac = (
+            tables.data_ptr(), ptr(acclim_x), ptr(td), ptr(sf), ptr(cf))
         rc = lib.ibis_step_ext(C.byref(ain), C.byref(aout), C.c_void_p(ws.data_ptr()),
                                C.c_int64(ws.numel()), slot_ptr, C.c_int64(state_in.pitch),
                                C.byref(p), C.byref(x), C.c_double(dtime),
@@ -203,11 +219,11 @@ def timestep_body(state: IBISState, consts: PhysicalConstants, params: SimParams
     flags = code | (abi.FLAG_BEDROCK if state.bedrock else 0)
     ws = workspace(state, code, torch.cuda.current_stream(state.device))
     p = ibis_params(state, consts, params)
-    if params.stomata_params.switches_on():
-        x, tables, acclim_x, td, sf = ext_inputs(state, params)
+    if uses_ext(params):
+        x, tables, acclim_x, td, sf, cf = ext_inputs(state, params)
         out = torch.ops.ibis.step_ext(state.arena, ws, state.slot_type, tables,
                                       abi.params_bytes(p), ops.ext_block(x), acclim_x, td, sf,
-                                      float(dtime), float(state.nytimes), int(state.n),
+                                      cf, float(dtime), float(state.nytimes), int(state.n),
                                       int(flags))
     else:
         out = torch.ops.ibis.step(state.arena, ws, state.slot_type,
@@ -219,9 +235,9 @@ def timestep_body(state: IBISState, consts: PhysicalConstants, params: SimParams
 
 
 def timestep_(state: IBISState, consts: PhysicalConstants, params: SimParams, dtime: float,
-              precision=None, stream=None, cparams=None) -> IBISState:
+              precision=None, stream=None, cparams=None, cext=None) -> IBISState:
     """One hour in place (the arena is read and written by the same launch)."""
     launch(state, state.arena, consts, params, dtime, precision=precision, stream=stream,
-           cparams=cparams)
+           cparams=cparams, cext=cext)
     state.nytimes = state.nytimes + 1.0
     return state

@@ -130,7 +130,8 @@ int ibis_step(const IbisArena* in, const IbisArena* out, void* workspace,
 #define IBIS_SW_ACCLIM_PER_PFT 0x100  /* acclim_per_pft: s0 / smax per plant type */
 #define IBIS_SW_ACCLIM_LIGHT 0x200    /* acclim_light: light-limited rate scaled too */
 #define IBIS_SW_FROST 0x400           /* frost_on: frost ramp on the day's mean temperature */
-#define IBIS_SW_ALL 0x7ff
+#define IBIS_SW_CROP 0x800            /* SimParams.cropfrac given: crop_vfac (loops.py:217-231) */
+#define IBIS_SW_ALL 0xfff
 
 /* Per-slot tables of ibis_step_ext, each IBIS_TYPE_ROWS doubles with the row rule of the
  * Vmax table: vmax_pft, coefm_pft, acclim_s0_pft, acclim_smax_pft (the per-cell (npoi, 12)
@@ -150,6 +151,10 @@ typedef struct IbisStomataExt {
   const void* stressfac;       /* per-cell stressfac, ncell elements, or NULL for
                                   IbisParams.stressfac (the per-zone scalar of
                                   python-port/ibis/io/pft_types.py:218-230) */
+  const void* cropfrac;        /* cropland share per cell, ncell elements; read with
+                                  IBIS_SW_CROP */
+  double crop_c4_share;        /* CropParams (python-port/ibis/io/parameters.py:262-275) */
+  double crop_vmax_scale_c3, crop_vmax_scale_c4;
 } IbisStomataExt;
 
 /* ibis_step with the optional StomataParams switches and per-slot tables of `ext` (device

@@ -64,8 +64,10 @@ def timestep_body(st, c, p, dtime, glob, flux_max=FLUX_MAX, t_max_dev=T_MAX_DEV)
     fira, raina, snowa = g("atm.fira"), g("atm.raina"), g("atm.snowa")
     lai, sai, fu, fl = g("vegetation.lai"), g("vegetation.sai"), g("vegetation.fu"), g("vegetation.fl")
     frac, exist = g("vegetation.frac"), g("vegetation.exist")
-    # acclimation factor of Vmax from the optional seasonal state (loops.py:291-310)
+    # acclimation and cropland factors of Vmax (loops.py:217-231, :291-310)
     vfac = stom_.acclim_factor(st.get("seasonal.acclim_x"), st.get("atm.td"), sp, frac.shape[0])
+    vfac = stom_.crop_factor(vfac, getattr(p, "cropfrac", None), getattr(p, "crop", None),
+                             frac.shape[0])
     stores = [g("canopy." + k) for k in ("wliqu", "wliqs", "wliql", "wsnou", "wsnos", "wsnol")]
 
     # 1. diagnostic soil properties (loops.py:318-326)

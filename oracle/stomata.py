@@ -3,7 +3,8 @@
 switches (``stomata.py:93-161``): composition-weighted Vmax (``vmax_mix``) and stomatal slope
 (``coefm_mix``), the leaf-only scaling denominator (``scale_leaf_only``), capacity scaling
 (``cap_scale_u/l``), the sunlit/shaded two-big-leaf canopy (``sunshade_u/l``), the
-acclimation factor of Vmax (``vfac``) and its light-rate factor (``acclim_light``).  A switch
+acclimation and cropland factors of Vmax (``vfac``) and the light-rate factor
+(``acclim_light``).  A switch
 is read with ``getattr(sp, name, False)``: a parameter object without it is the F77 path.
 
 Pinned by: ``colimit`` against the smaller root of theta x^2 - (j1+j2) x + j1 j2 from the
@@ -236,6 +237,24 @@ def acclim_factor(acclim_x, td, sp, n):
                      / (sp.frost_t1 - sp.frost_t0), 0.0, 1.0)
         v = v * (1.0 - mask[None, :] + mask[None, :] * ff[:, None])
     return v
+
+
+def crop_factor(vfac, cropfrac, crop, n):
+    """The cropland factor on the grasses' Vmax (timestepping/loops.py:217-231): 1 + cropfrac x
+    (scale - 1), the C4 scale on PFT 10 and the C3 scale on PFT 11 (c4_share < 0), or both the
+    c4_share mix of the two; composed with the acclimation factor.  ``vfac`` unchanged without
+    a cropland share."""
+    if cropfrac is None:
+        return vfac
+    m4, m3 = np.zeros(12), np.zeros(12)
+    m4[10], m3[11] = 1.0, 1.0
+    q = crop.c4_share
+    qc = min(max(q, 0.0), 1.0)
+    s_mix = (1.0 - qc) * crop.vmax_scale_c3 + qc * crop.vmax_scale_c4
+    per_pft = (crop.vmax_scale_c4 - 1.0) * m4 + (crop.vmax_scale_c3 - 1.0) * m3
+    by_share = (s_mix - 1.0) * (m4 + m3)
+    g = 1.0 + np.asarray(cropfrac, dtype=np.float64)[:, None] * (per_pft if q < 0.0 else by_share)[None, :]
+    return g if vfac is None else vfac * g
 
 
 def drystress(wsoi, wisoi, swilt, sfield, froot, stressfac):

@@ -107,7 +107,8 @@ def host_step_ext(f, slot, prec, params, aux=None, bedrock=True, consts=None, dt
     sp = params.stomata_params
     x = abi.stomata_ext(sp, tab.ctypes.data, plane(aux.get("seasonal.acclim_x")),
                         plane(aux.get("atm.td")),
-                        plane(sp.stressfac if np.ndim(sp.stressfac) else None))
+                        plane(sp.stressfac if np.ndim(sp.stressfac) else None),
+                        plane(getattr(params, "cropfrac", None)), getattr(params, "crop", None))
     if "seasonal.acclim_x" not in aux:
         x.switches &= ~abi.SW_ACCLIM
     flags = PREC_CODE[prec] | (abi.FLAG_BEDROCK if bedrock else 0)
@@ -152,6 +153,17 @@ def capscale_params():
     sp.scale_leaf_only = True
     sp.acclim_on = sp.acclim_light = sp.frost_on = True
     sp.cap_kn = 0.2
+    return sim
+
+
+def crop_params(n, seed=9):
+    """Cropland on a random share of each cell with the release file's crop scales
+    (params_31s_ch4h.yaml:427-431) and the light-rate factor, acclimation off."""
+    sim = default_sim_params()
+    sim.cropfrac = np.random.default_rng(seed).uniform(0.0, 1.0, n)
+    sim.crop.vmax_scale_c3, sim.crop.vmax_scale_c4, sim.crop.c4_share = 0.3293, 1.1583, 0.2
+    sim.stomata_params.acclim_light = True
+    sim.stomata_params.vmax_mix = True
     return sim
 
 

@@ -11,8 +11,9 @@ from ibis_amd.state import IBISState
 from ibis_amd.step import timestep_, timestep_body
 from oracle import params as OP
 
-from _helpers import (assert_parity, capscale_params, host_step_ext, oracle_with_band,
-                      release_params, rounded, seasonal, workload)
+from _helpers import (assert_parity, host_step_ext, oracle_with_band, release_params, rounded,
+                      seasonal, workload)
+from test_release_host import switch_config
 
 pytestmark = pytest.mark.gpu
 
@@ -24,21 +25,13 @@ def _state(f, slot, prec, aux=None):
                                 device="cuda", dtype=DT[prec])
 
 
-def _params(config, n, prec):
-    sim = release_params(n) if config == "release" else capscale_params()
-    if prec != "f64" and np.ndim(sim.stomata_params.stressfac):
-        sim.stomata_params.stressfac = np.asarray(sim.stomata_params.stressfac,
-                                                  np.float32).astype(np.float64)
-    return sim
-
-
 @pytest.mark.parametrize("prec", ["f64", "mixed"])
-@pytest.mark.parametrize("config", ["release", "capscale"])
+@pytest.mark.parametrize("config", ["release", "capscale", "crop"])
 def test_switches_match_the_oracle_and_the_host_build(prec, config):
     n = 4000
     print(f"\nrelease gpu: {config} {prec} n={n}", flush=True)
     f, slot = workload(n, seed=81, hour=13, prec=prec)
-    sim = _params(config, n, prec)
+    sim = switch_config(config, n, prec)
     aux = rounded(seasonal(n), prec)
     band, refs, keys, tol = oracle_with_band(dict(f, **aux), slot, prec, params=sim)
     new = timestep_body(_state(f, slot, prec, aux), get_constants(), sim, 3600.0,

@@ -298,3 +298,20 @@ def test_cap_scale_with_kn_equal_to_the_light_profile_is_the_f77_canopy():
     np.testing.assert_allclose(on.ag_upper, off.ag_upper, rtol=1e-8)
     np.testing.assert_allclose(on.gs_upper, off.gs_upper, rtol=1e-8)
     np.testing.assert_array_equal(on.ag_lower, off.ag_lower)
+
+
+def test_cropland_factor_scales_only_the_grasses():
+    crop = SimpleNamespace(c4_share=-1.0, vmax_scale_c3=0.33, vmax_scale_c4=1.16)
+    cf = np.array([0.0, 0.5, 1.0])
+    g = ST.crop_factor(None, cf, crop, 3)
+    np.testing.assert_array_equal(g[:, :10], 1.0)
+    np.testing.assert_allclose(g[:, 10], 1.0 + cf * 0.16, rtol=1e-15)
+    np.testing.assert_allclose(g[:, 11], 1.0 - cf * 0.67, rtol=1e-15)
+    crop.c4_share = 0.2                              # both grasses take the mix of the scales
+    g2 = ST.crop_factor(None, cf, crop, 3)
+    mix = 0.8 * 0.33 + 0.2 * 1.16
+    np.testing.assert_allclose(g2[:, 10], 1.0 + cf * (mix - 1.0), rtol=1e-15)
+    np.testing.assert_array_equal(g2[:, 10], g2[:, 11])
+    v = np.full((3, 12), 0.5)
+    np.testing.assert_allclose(ST.crop_factor(v, cf, crop, 3), 0.5 * g2, rtol=1e-15)
+    assert ST.crop_factor(v, None, crop, 3) is v

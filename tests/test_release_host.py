@@ -8,8 +8,8 @@ import copy
 import numpy as np
 import pytest
 
-from _helpers import (assert_parity, capscale_params, host_step, host_step_ext, oracle_with_band,
-                      release_params, rounded, seasonal, workload)
+from _helpers import (assert_parity, capscale_params, crop_params, host_step, host_step_ext,
+                      oracle_with_band, release_params, rounded, seasonal, workload)
 from ibis_amd import abi
 from ibis_amd.params import default_sim_params
 
@@ -24,15 +24,24 @@ def test_no_switch_is_the_plain_step_bit_for_bit():
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"{prec} {k}")
 
 
+def switch_config(config, n, prec):
+    sim = {"release": lambda: release_params(n), "capscale": capscale_params,
+           "crop": lambda: crop_params(n)}[config]()
+    if prec != "f64":                            # the per-cell planes as the FP32 build sees them
+        if np.ndim(sim.stomata_params.stressfac):
+            sim.stomata_params.stressfac = np.asarray(sim.stomata_params.stressfac,
+                                                      np.float32).astype(np.float64)
+        if sim.cropfrac is not None:
+            sim.cropfrac = np.asarray(sim.cropfrac, np.float32).astype(np.float64)
+    return sim
+
+
 @pytest.mark.parametrize("prec", ["f64", "mixed"])
-@pytest.mark.parametrize("config", ["release", "capscale"])
+@pytest.mark.parametrize("config", ["release", "capscale", "crop"])
 def test_switches_match_the_oracle(prec, config):
     n = 1500
     f, slot = workload(n, seed=61, hour=13, prec=prec)
-    sim = release_params(n) if config == "release" else capscale_params()
-    if prec != "f64" and np.ndim(sim.stomata_params.stressfac):
-        sim.stomata_params.stressfac = np.asarray(sim.stomata_params.stressfac,
-                                                  np.float32).astype(np.float64)
+    sim = switch_config(config, n, prec)
     aux = rounded(seasonal(n), prec)
     fo = dict(f, **aux)
     band, refs, keys, tol = oracle_with_band(fo, slot, prec, params=sim)
@@ -45,8 +54,8 @@ def test_switches_change_the_canopy():
     f, slot = workload(n, seed=71, hour=13, prec="f64")
     base = host_step(f, slot, "f64")
     aux = seasonal(n)
-    for config in ("release", "capscale"):
-        sim = release_params(n) if config == "release" else capscale_params()
+    for config in ("release", "capscale", "crop"):
+        sim = switch_config(config, n, "f64")
         got = host_step_ext(f, slot, "f64", sim, aux)
         day = base["canopy.ag_upper"][:, 0] > 0
         assert day.sum() > 50
@@ -85,8 +94,10 @@ def test_ext_entry_point_rejects_bad_switch_blocks():
         C.c_int64(4), C.c_int(flags), None)
     assert call(None) == -1
     bad = abi.IbisStomataExt.from_buffer_copy(x)
-    bad.switches = 0x800
+    bad.switches = 0x1000
     assert call(bad) == -2 and b"switch" in lib.ibis_last_error()
+    bad.switches = abi.SW_CROP
+    assert call(bad) == -1 and b"cropland" in lib.ibis_last_error()
     bad = abi.IbisStomataExt.from_buffer_copy(x)
     bad.switches = abi.SW_ACCLIM
     assert call(bad) == -1 and b"acclimation state" in lib.ibis_last_error()
