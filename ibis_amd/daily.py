@@ -324,6 +324,7 @@ def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR
         d = day0 + k
         p.jday = int(weather.jday0 + d)
         state.aux["atm.td"] = w[0]            # the day's mean temperature (frost ramp)
+        cext = S.ext_inputs(state, params) if S.uses_ext(params) else None
         day_begin(state, day, w, acc, p, stream=st)
         wa = _arena(w)
         for h in range(nstep):
@@ -335,7 +336,7 @@ def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR
                                           C.c_void_p(st.cuda_stream))
             _lib.check(rc, "ibis_forcing_diurnal")
             S.timestep_(state, consts, params, dtime, precision=precision, stream=st,
-                        cparams=cparams)
+                        cparams=cparams, cext=cext)
             acc.add(state, stream=st)
         day_end(state, day, w, acc, p, stream=st)
         if on_day is not None:

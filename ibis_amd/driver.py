@@ -210,9 +210,11 @@ def run_hours(state: IBISState, hours: int, forcing, consts, params, dtime: floa
     launches (e.g. to record events)."""
     st = _stream(state, stream)
     cparams = S.ibis_params(state, consts, params)
+    cext = S.ext_inputs(state, params) if S.uses_ext(params) else None
     for h in range(hour0, hour0 + hours):
         forcing.write(state, h, stream=st)
-        S.timestep_(state, consts, params, dtime, precision=precision, stream=st, cparams=cparams)
+        S.timestep_(state, consts, params, dtime, precision=precision, stream=st, cparams=cparams,
+                    cext=cext)
         if acc is not None:
             acc.add(state, stream=st)
         if on_hour is not None:
