@@ -17,7 +17,7 @@ EXPORTS = ("ibis_step", "ibis_step_ext", "ibis_step_workspace_bytes", "ibis_last
            "ibis_version",
            "ibis_forcing_synthetic", "ibis_forcing_diurnal", "ibis_accumulate",
            "ibis_serve_tick", "ibis_tuned_workspace_bytes", "ibis_tuned_hour",
-           "ibis_day_begin", "ibis_day_end")
+           "ibis_day_begin", "ibis_day_end", "ibis_step_cpu")
 
 _lib = None
 
@@ -44,6 +44,10 @@ def load():
                                   C.POINTER(IbisStomataExt), C.c_double, C.c_double, C.c_int64,
                                   C.c_int, C.c_void_p]
     lib.ibis_step_ext.restype = C.c_int
+    lib.ibis_step_cpu.argtypes = [C.POINTER(IbisArena), C.POINTER(IbisArena), C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.POINTER(IbisParams), C.c_double,
+                                  C.c_double, C.c_int64, C.c_int, C.c_int]
+    lib.ibis_step_cpu.restype = C.c_int
     lib.ibis_step_workspace_bytes.argtypes = [C.c_int64, C.c_int]
     lib.ibis_step_workspace_bytes.restype = C.c_int64
     lib.ibis_last_error.restype = C.c_char_p

@@ -5,6 +5,8 @@
 * ``ibis_amd/libibis_step.so``       -- the product: HIP kernels + C ABI (the step,
   ``include/ibis_step.h``, and the hourly driver, ``include/ibis_driver.h``),
   ``--offload-arch=gfx950``;
+* ``ibis_cpu.o`` in ``libibis_step.so`` -- the host CPU path ``ibis_step_cpu`` (the same cell
+  source with g++ over host threads);
 * ``ibis_amd/libibis_hostcheck.so``  -- the same step source built for the host with g++,
   loaded only by tests to check the kernel source against the oracle without a GPU.
 
@@ -78,7 +80,15 @@ def build(verbose: bool = False):
     workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
         msg = "".join(pool.map(_run, jobs))
-    msg += _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
+    # the host CPU path (include/ibis_cpu.h): the same cell source with g++, x86-64-v3 so that
+    # the object runs on any AVX2 host, contraction off as in the kernels
+    cpu_obj = os.path.join(objdir, "ibis_cpu.o")
+    msg += _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+                 "-Wno-unknown-pragmas", f"-I{INC}", "-c", os.path.join(CSRC, "ibis_cpu.cpp"),
+                 "-o", cpu_obj])
+    objs.append(cpu_obj)
+    msg += _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lpthread", "-o",
+                 out + ".tmp"])
     os.replace(out + ".tmp", out)
     hout = os.path.join(HERE, "libibis_hostcheck.so")
     msg2 = _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "ibis_hostcheck.cpp"), "-o", hout + ".tmp"])
