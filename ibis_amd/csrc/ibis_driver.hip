@@ -51,6 +51,28 @@ __global__ __launch_bounds__(kDriverBlock) void accumulate_kernel(
   accumulate_cell(st, pitch, vx, vpitch, acc, apitch, i, p);
 }
 
+// One thread per cell: whether any written plane is NaN or Inf; the wave's count is added by
+// its first active lane (a vector atomic on global memory).
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void count_nonfinite_kernel(
+    const S* st, int64_t pitch, int64_t ncell, unsigned long long* counter) {
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  bool bad = false;
+  if (i < ncell) {
+#define X(F, W, OFF, ROLE)                                                      \
+  if ((ROLE) != IBIS_ROLE_IN)                                                   \
+    for (int j = 0; j < (W); ++j) {                                             \
+      const S v = st[(int64_t)((OFF) + j) * pitch + i];                         \
+      bad = bad || !(v - v == S(0));                                            \
+    }
+    IBIS_FIELD_TABLE
+#undef X
+  }
+  const unsigned long long mask = __ballot(bad);
+  if (mask != 0 && (threadIdx.x % warpSize) == (unsigned)(__ffsll((long long)__ballot(1)) - 1))
+    atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
 static int launch_error(const char* what) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -156,6 +178,29 @@ int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisAre
                        state->pitch, (const float*)vegx->planes, vegx->pitch,
                        (float*)acc->planes, acc->pitch, ncell, *p);
   return launch_error("ibis_accumulate");
+}
+
+int ibis_count_nonfinite(const IbisArena* state, int64_t ncell, int flags,
+                         unsigned long long* counter, void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0 || !counter)
+    return fail(IBIS_E_ARG, "ibis_count_nonfinite: negative size or null counter");
+  if (check_arena(state, ncell, "ibis_count_nonfinite: null state or pitch < ncell"))
+    return IBIS_E_ARG;
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~(IBIS_PREC_MASK | IBIS_FLAG_BEDROCK | IBIS_FLAG_COPY_IN)) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_count_nonfinite: unknown flag bits or precision");
+  if (ncell == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(count_nonfinite_kernel<double>, grid, block, 0, s,
+                       (const double*)state->planes, state->pitch, ncell, counter);
+  else
+    hipLaunchKernelGGL(count_nonfinite_kernel<float>, grid, block, 0, s,
+                       (const float*)state->planes, state->pitch, ncell, counter);
+  return launch_error("ibis_count_nonfinite");
 }
 
 }  // extern "C"

@@ -89,6 +89,9 @@ def _lib_driver():
                                         C.POINTER(abi.IbisArena), C.POINTER(IbisAccParams),
                                         C.c_int64, C.c_int, C.c_void_p]
         lib.ibis_accumulate.restype = C.c_int
+        lib.ibis_count_nonfinite.argtypes = [C.POINTER(abi.IbisArena), C.c_int64, C.c_int,
+                                             C.c_void_p, C.c_void_p]
+        lib.ibis_count_nonfinite.restype = C.c_int
         lib._driver_typed = True
     return lib
 
@@ -200,6 +203,21 @@ class Accumulators:
         """Compensated totals (sum minus the pending compensation), float64 numpy (N,)."""
         tot = (self.acc[:NACC].double() - self.acc[NACC:].double())[:, :self.n].cpu().numpy()
         return {nm: tot[j] for j, nm in enumerate(ACC_NAMES)}
+
+
+def count_nonfinite(state: IBISState, counter: torch.Tensor | None = None, stream=None):
+    """Enqueue a count of the cells with a non-finite value in any plane the step writes
+    (``ibis_count_nonfinite``); returns the device counter (int64, one element), to be read
+    whenever the caller synchronises.  A given ``counter`` is added to, not zeroed."""
+    lib = _lib_driver()
+    st = _stream(state, stream)
+    if counter is None:
+        counter = torch.zeros(1, dtype=torch.int64, device=state.device)
+    a = abi.IbisArena(state.arena.data_ptr(), state.pitch)
+    rc = lib.ibis_count_nonfinite(C.byref(a), C.c_int64(state.n), C.c_int(_code(state)),
+                                  C.c_void_p(counter.data_ptr()), C.c_void_p(st.cuda_stream))
+    _lib.check(rc, "ibis_count_nonfinite")
+    return counter
 
 
 def run_hours(state: IBISState, hours: int, forcing, consts, params, dtime: float = 3600.0,

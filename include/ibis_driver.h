@@ -99,6 +99,14 @@ typedef struct IbisAccParams {
 int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisArena* acc,
                     const IbisAccParams* p, int64_t ncell, int flags, void* hip_stream);
 
+/* Count the cells whose state after a step holds a non-finite value in any plane the step
+ * writes (IO and OUT planes of ibis_layout.h), adding the count to *counter (device memory,
+ * one unsigned 64-bit word; the caller zeroes it).  The reference does not stop on NaN or Inf
+ * either (its clamps bound the results, loops.py:665-674, :1143-1151); this reports them
+ * without a host synchronisation.  flags: the precision bits of ibis_step. */
+int ibis_count_nonfinite(const IbisArena* state, int64_t ncell, int flags,
+                         unsigned long long* counter, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,3 +129,15 @@ def test_synthetic_cells_are_batch_independent():
     np.testing.assert_array_equal(s1[60:], s2)
     for k in f1:
         np.testing.assert_array_equal(np.asarray(f1[k])[60:], f2[k])
+
+
+def test_nonfinite_counter_rejects_bad_arguments():
+    from ibis_amd import driver as DR
+    lib = DR._lib_driver()
+    a = abi.IbisArena(C.c_void_p(256), 8)
+    call = lambda n, flags, counter=256: lib.ibis_count_nonfinite(
+        C.byref(a), C.c_int64(n), C.c_int(flags), C.c_void_p(counter), None)
+    assert call(9, abi.PREC_MIXED) == -1                     # pitch < ncell
+    assert call(4, abi.PREC_MIXED, counter=0) == -1
+    assert call(4, abi.PREC_MIXED | 0x1000) == -2
+    assert call(0, abi.PREC_MIXED) == 0

@@ -9,7 +9,9 @@ import torch
 from ibis_amd import driver as DR, synth
 from ibis_amd.params import default_sim_params, get_constants
 from ibis_amd.state import IBISState
+from ibis_amd.step import timestep_
 from oracle import driver as OD, params as OP
+from _helpers import workload
 
 from test_driver import _acc_case, _daily
 
@@ -130,3 +132,21 @@ def test_thirty_days_mixed_precision_against_an_fp64_gpu_run():
             assert e.max() < 1e-6, k          # forcing only: FP32 storage of the same draws
         else:
             assert np.median(e) < 1e-3, k
+
+
+def test_nonfinite_counter_counts_cells_with_a_bad_written_plane():
+    n = 3000
+    f, slot = workload(n, seed=7, hour=12, prec="mixed")
+    st = IBISState.from_numpy(f, slot, synth.HSOI, OP.root_fractions(), device="cuda")
+    new = timestep_(st, get_constants(), default_sim_params(), 3600.0)
+    c = DR.count_nonfinite(new)
+    torch.cuda.synchronize()
+    assert int(c.item()) == 0
+    new.canopy.tu[[5, 17]] = float("nan")                 # written planes: counted per cell
+    new.soil.tsoi[17, 3] = float("inf")
+    new.hydrology.gtrans[2999] = float("-inf")
+    new.soil.sand[40, 0] = float("nan")                   # a read-only plane: not counted
+    c = DR.count_nonfinite(new)
+    DR.count_nonfinite(new, counter=c)                    # adds to the given counter
+    torch.cuda.synchronize()
+    assert int(c.item()) == 2 * 3
