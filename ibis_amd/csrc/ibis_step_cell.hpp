@@ -464,10 +464,12 @@ IBIS_HD void big_leaves(StomExt<R>& X, const Cell<S>& c, const StepExt& E,
   X.sw = E.sw;
 }
 
-template <class S, class R, class D, class PT, bool EXT>
+// FLDS: the turvap inputs F live in the caller's per-thread LDS slot `fslot` and are read
+// from there in each pass instead of being held in registers across the three passes.
+template <class S, class R, class D, class PT, bool EXT, bool FLDS = false>
 IBIS_HD void canopy_cell_x(const Cell<S>& c, const Work<R, D>& wk, const R vrow[IBIS_NPFT],
                            const PT& P, const StepConst& K, const StepExt* E,
-                           const R (*xrow)[IBIS_NPFT]) {
+                           const R (*xrow)[IBIS_NPFT], TvFix<R, D>* fslot = nullptr) {
   const R eps = R(P.epsilon);
   const D dtD = D(K.dtime), tmeltD = D(P.tmelt);
 
@@ -548,7 +550,11 @@ IBIS_HD void canopy_cell_x(const Cell<S>& c, const Work<R, D>& wk, const R vrow[
 
   const R hsoi0 = R(P.hsoi[0]);
 
-  TvFix<R, D> F;
+  TvFix<R, D> Floc;
+  TvFix<R, D>& F = [&]() -> TvFix<R, D>& {
+    if constexpr (FLDS) return *fslot;
+    else return Floc;
+  }();
   {
     const R cog = consoi0 / mx(R(0.5) * hsoi0, eps);
     const R coi = R(P.consno) / mx(R(0.5) * mx(hsno0, R(P.hsnotop)), eps);
@@ -630,6 +636,14 @@ IBIS_HD void canopy_cell_x(const Cell<S>& c, const Work<R, D>& wk, const R vrow[
   const R co2 = R(P.co2conc), zw = R(K.zweight);
   const D chgxD = chgx, wlgxD = wlgx, wigxD = wigx;
   for (int it = 0; it < 3; ++it) {
+    const TvFix<R, D>* fp = &F;
+    if constexpr (FLDS) {
+      // an opaque copy of the LDS address per pass keeps the reads inside the loop
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" : "+v"(fp));
+#endif
+    }
+    const TvFix<R, D>& FF = *fp;
     const bool first = it == 0;
     Turcof<D> tc;
     turcof(tc, aero, D(K.tfac), taD, qaD, uaD, cur[3], cur[4], cur[5], cur[6], dtD, first, u2c,
@@ -638,16 +652,16 @@ IBIS_HD void canopy_cell_x(const Cell<S>& c, const Work<R, D>& wk, const R vrow[
                              R(cur[6]), R(tc.su), R(tc.sl), psurf, rad, fw2.fwetu, fw2.fwetl,
                              lai, sai, EXT ? X.vnom : vnom, stresstu, stresstl, co2, zw, P, &X);
     if (!first) {
-      wp[0] = impexp(cur[0], D(F.chux), D(F.w[0]), D(F.w[3]), wp[0], P);
-      wp[1] = impexp(cur[1], D(F.chsx), D(F.w[1]), D(F.w[4]), wp[1], P);
-      wp[2] = impexp(cur[2], D(F.chlx), D(F.w[2]), D(F.w[5]), wp[2], P);
+      wp[0] = impexp(cur[0], D(FF.chux), D(FF.w[0]), D(FF.w[3]), wp[0], P);
+      wp[1] = impexp(cur[1], D(FF.chsx), D(FF.w[1]), D(FF.w[4]), wp[1], P);
+      wp[2] = impexp(cur[2], D(FF.chlx), D(FF.w[2]), D(FF.w[5]), wp[2], P);
       wp[3] = impexp(cur[7], chgxD, wlgxD, wigxD, wp[3], P);
-      wp[4] = impexp2(cur[8], D(F.tiold), wp[4], P);
+      wp[4] = impexp2(cur[8], D(FF.tiold), wp[4], P);
     }
     D tcD[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) tcD[j] = sm.tcond[j];
-    turvap(tv, F, cur, tc.cu, tc.cl, tc.su, tc.ss, tc.sl, tc.sg, tc.si, tcD, wp[0], wp[1], wp[2], wp[3], wp[4], dtD, P);
+    turvap(tv, FF, cur, tc.cu, tc.cl, tc.su, tc.ss, tc.sl, tc.sg, tc.si, tcD, wp[0], wp[1], wp[2], wp[3], wp[4], dtD, P);
 #pragma unroll
     for (int j = 0; j < 9; ++j) cur[j] = tv.t[j];
     IBIS_TRACE(c.i, "si", tc.si);

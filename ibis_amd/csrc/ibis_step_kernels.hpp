@@ -52,6 +52,7 @@ __global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel(
     const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, const int8_t* slot,
     int64_t slot_pitch, const double* table, IbisParams P, StepConst K) {
   __shared__ R trow[IBIS_TYPE_ROWS];
+  __shared__ TvFix<R, D> fslot[kBlock];
   for (int t = threadIdx.x; t < IBIS_TYPE_ROWS; t += blockDim.x) trow[t] = R(table[t]);
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -64,7 +65,8 @@ __global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel(
     vrow[j] = trow[row];
   }
   const Cell<S> c{in, out, pitch, i};
-  canopy_cell<S, R, D>(c, work_at<R, D>(ws, pitch, i), vrow, P, K);
+  canopy_cell_x<S, R, D, IbisParams, false, true>(c, work_at<R, D>(ws, pitch, i), vrow, P, K,
+                                                  nullptr, nullptr, &fslot[threadIdx.x]);
 }
 
 template <class S, class R, class D>
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel_ext(
     const S* in, S* out, void* ws, int64_t pitch, int64_t ncell, const int8_t* slot,
     int64_t slot_pitch, const double* tables, IbisParams P, StepConst K, StepExt E) {
   __shared__ R trow[IBIS_NTABLE][IBIS_TYPE_ROWS];
+  __shared__ TvFix<R, D> fslot[kBlock];
   for (int t = threadIdx.x; t < IBIS_NTABLE * IBIS_TYPE_ROWS; t += blockDim.x)
     trow[t / IBIS_TYPE_ROWS][t % IBIS_TYPE_ROWS] = R(tables[t]);
   __syncthreads();
@@ -119,7 +122,8 @@ __global__ __launch_bounds__(kBlock, kWavesCanopy) void canopy_kernel_ext(
     for (int q = 0; q < 3; ++q) xrow[q][j] = trow[1 + q][row];
   }
   const Cell<S> c{in, out, pitch, i};
-  canopy_cell_x<S, R, D, IbisParams, true>(c, work_at<R, D>(ws, pitch, i), vrow, P, K, &E, xrow);
+  canopy_cell_x<S, R, D, IbisParams, true, true>(c, work_at<R, D>(ws, pitch, i), vrow, P, K, &E,
+                                                 xrow, &fslot[threadIdx.x]);
 }
 
 template <class S, class R, class D>
