@@ -12,8 +12,8 @@ from ibis_amd.cpu import HostStep
 
 @pytest.mark.parametrize("prec", ["f64", "mixed"])
 def test_cpu_path_matches_the_oracle(prec):
-    """FP64 to 1e-6 and mixed to the 1e-4 bar; FP32 throughout (the config-0 precision) is
-    measured, not held to the bar, as on the GPU."""
+    """FP64 to 1e-6 and mixed to the 1e-4 bar; FP32 throughout (the precision of the
+    1,024-cell CPU case) is measured, not held to the bar, as on the GPU."""
     n = 1024
     f, slot = workload(n, seed=91, hour=13, prec=prec)
     band, refs, keys, tol = oracle_with_band(f, slot, prec)
