@@ -64,6 +64,8 @@ NODE = {"soil": "SoilState", "snow": "SnowState", "canopy": "CanopyState", "atm"
 #: IBISState's sub-states in field order; the optional modules follow
 #: (permafrost, wetland, fire, bvoc, stress, etparts, seasonal, neeparts)
 SUBSTATES = ("soil", "snow", "canopy", "atm", "vegetation", "carbon", "hydrology", "summary")
+#: SeasonalState's leaves in field order (python-port/ibis/state/seasonal.py:36-43)
+SEASONAL_FIELDS = ("acclim_x", "ssen_u", "ssen_l")
 #: leaf counts of the optional trailing chain, and of its first module in old files
 CHAIN_LEAVES = (4, 4, 3, 2)
 OLD_STRESS_LEAVES = 3
@@ -162,6 +164,11 @@ class Restart:
             start += n
         self.trailing = [(k[1] if k[0] == "node" else None, count_leaves(k))
                          for k in kids[len(SUBSTATES):]]
+        self.module_offset = {}
+        for name, n in self.trailing:
+            if name is not None:
+                self.module_offset[name] = start
+            start += n
         self._check_chain()
         self.ncell = int(np.asarray(self.leaf("soil.tg")).shape[0])
 
@@ -238,6 +245,18 @@ class Restart:
             out[key] = a
         return out
 
+    def aux_fields(self) -> dict:
+        """The optional per-cell leaves the step's switches read, where the file has them:
+        the day's mean temperature ``atm.td`` and, in a file carrying the SeasonalState module
+        (state/seasonal.py:36-43), ``seasonal.acclim_x``, ``ssen_u``, ``ssen_l``."""
+        out = {"atm.td": np.asarray(self.leaf("atm.td"), dtype=np.float64).reshape(self.ncell)}
+        o = self.module_offset.get("SeasonalState")
+        if o is not None:
+            for j, k in enumerate(SEASONAL_FIELDS):
+                out[f"seasonal.{k}"] = np.asarray(self.leaves[o + j],
+                                                  dtype=np.float64).reshape(self.ncell)
+        return out
+
     def to_state(self, slot_type=None, *, device="cuda", dtype=torch.float32, pitch=None,
                  bedrock=None):
         """An ``IBISState`` holding the file's step fields.  The snow constants come from the
@@ -250,7 +269,8 @@ class Restart:
         nyt = np.asarray(self.leaf("vegetation.nytimes"), dtype=np.float64).reshape(-1)[0]
         if bedrock is None:
             bedrock = self.leaf("soil.tsoi_deep") is not None
-        return IBISState.from_numpy(self.fields(), slot_type, hsoi, froot, device=device,
+        return IBISState.from_numpy({**self.fields(), **self.aux_fields()}, slot_type, hsoi,
+                                    froot, device=device,
                                     dtype=dtype, pitch=pitch, co2conc=float(co2),
                                     nytimes=float(nyt), snow_const=SnowConstants(),
                                     bedrock=bool(bedrock))
@@ -268,3 +288,10 @@ class Restart:
         i = self.index("vegetation.nytimes")
         old = np.asarray(self.leaves[i])
         self.leaves[i] = np.full(old.shape, state.nytimes, dtype=old.dtype)
+        o = self.module_offset.get("SeasonalState")
+        for j, k in enumerate(SEASONAL_FIELDS):
+            t = state.aux.get(f"seasonal.{k}")
+            if o is not None and t is not None:
+                old = self.leaves[o + j]
+                v = t[:state.n].detach().cpu().numpy()
+                self.leaves[o + j] = np.asarray(v).reshape(np.shape(old)).astype(old.dtype)

@@ -150,3 +150,29 @@ def test_one_hour_from_a_reference_restart_on_the_host_build(tmp_path):
         np.testing.assert_array_equal(np.asarray(r2.leaf(k), dtype=np.float64),
                                       np.asarray(out[k], dtype=np.float32).astype(np.float64),
                                       err_msg=k)
+
+
+def test_seasonal_states_map_to_the_state_and_back(tmp_path):
+    """A checkpoint carrying the SeasonalState module: its three leaves become the state's
+    seasonal.* leaves beside the arena (with atm.td), and update() writes them back."""
+    mods = (("StressAccum", 4), ("ETParts", 4), ("SeasonalState", 3))
+    nodes = ["None"] * (8 - len(mods)) + [
+        f"CustomNode({m}[(Missing,)], [{', '.join(['*'] * c)}])" for m, c in mods]
+    tree, r = _tree_with(nodes)
+    extra = [np.zeros(4, np.float32)] * 8 + [np.array([1.5, -2.0, 7.25, 0.0], np.float32),
+                                             np.array([0.0, 12.0, 0.0, 3.0], np.float32),
+                                             np.array([1.0, 0.0, 2.0, 0.0], np.float32)]
+    rr = RS.Restart(r.leaves + extra, tree, r.meta)
+    aux = rr.aux_fields()
+    np.testing.assert_array_equal(aux["seasonal.acclim_x"], [1.5, -2.0, 7.25, 0.0])
+    np.testing.assert_array_equal(aux["atm.td"], np.asarray(rr.leaf("atm.td"), np.float64))
+    st = rr.to_state(device="cpu")
+    np.testing.assert_array_equal(st.seasonal.ssen_u.numpy(), [0.0, 12.0, 0.0, 3.0])
+    st.seasonal.acclim_x = np.array([9.0, 8.0, 7.0, 6.0])
+    rr.update(st)
+    p = str(tmp_path / "s.npz")
+    rr.save(p)
+    back = RS.Restart.load(p)
+    np.testing.assert_array_equal(back.aux_fields()["seasonal.acclim_x"], [9.0, 8.0, 7.0, 6.0])
+    assert back.leaves[-3].dtype == np.float32
+    assert "seasonal.acclim_x" not in RS.Restart.load(FIX).aux_fields()
