@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kDriverBlock) void forcing_synthetic_kernel(
   put_forcing(a, pitch, i, f);
 }
 
-template <class S>
+template <class S, bool OBS_LW>
 __global__ __launch_bounds__(kDriverBlock) void forcing_diurnal_kernel(
     S* a, int64_t pitch, const S* daily, int64_t dpitch, int64_t ncell, const double* lat,
     double time, double jday, double dtime, uint64_t seed, int64_t cell0, int64_t step) {
@@ -38,8 +38,44 @@ __global__ __launch_bounds__(kDriverBlock) void forcing_diurnal_kernel(
   Forcing f;
   double ps;
   diurnal_forcing(f, ps, dw, lat[i], time, jday, dtime, rn);
+  if (OBS_LW) f.fira = (double)daily[(int64_t)IBIS_DAILY_RLDS * dpitch + i];
   put_forcing(a, pitch, i, f);
   a[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = S(ps);
+}
+
+// One thread per cell and blockIdx.y = a chunk of kDayChunk days: the day terms of the solar
+// geometry are computed once per block into LDS and the latitude terms once per thread, so a
+// cell-day costs two transcendentals (the sunset hour angle and its sine) besides its
+// IBIS_NGSWP3 float32 reads and IBIS_NDAILY_OBS stores, unit stride across the wavefront.
+constexpr int kDayChunk = 8;
+
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void gswp3_days_kernel(
+    const float* obs, int64_t opitch, int ndays, int jday0, const double* lat, int64_t ncell,
+    double dtime, uint64_t seed, int64_t cell0, int64_t day0, S* daily, int64_t dpitch) {
+  __shared__ SunDay sun[kDayChunk];
+  const int dfirst = (int)blockIdx.y * kDayChunk;
+  const int nd = ndays - dfirst < kDayChunk ? ndays - dfirst : kDayChunk;
+  if ((int)threadIdx.x < nd) sun[threadIdx.x] = sun_day((double)(jday0 + dfirst + (int)threadIdx.x));
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  if (i >= ncell) return;
+  const SunLat L = sun_lat(lat[i]);
+  const uint64_t c = (uint64_t)(cell0 + i);
+  for (int k = 0; k < nd; ++k) {
+    const int d = dfirst + k;
+    double o[IBIS_NGSWP3];
+#pragma unroll
+    for (int v = 0; v < IBIS_NGSWP3; ++v)
+      o[v] = (double)obs[((int64_t)v * ndays + d) * opitch + i];
+    const uint64_t day = (uint64_t)(day0 + d);
+    double w[IBIS_NDAILY_OBS];
+    gswp3_day<S>(w, o, L, sun[k], dtime, uniform01(seed, kStreamEventLen, c, day),
+                 uniform01(seed, kStreamEventStart, c, day));
+    S* out = daily + (int64_t)d * IBIS_NDAILY_OBS * dpitch + i;
+#pragma unroll
+    for (int j = 0; j < IBIS_NDAILY_OBS; ++j) out[(int64_t)j * dpitch] = S(w[j]);
+  }
 }
 
 template <class S>
@@ -132,21 +168,61 @@ int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
     return IBIS_E_ARG;
   if (!lat_deg) return fail(IBIS_E_ARG, "ibis_forcing_diurnal: null latitude array");
   const int prec = flags & IBIS_PREC_MASK;
-  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+  if ((flags & ~(IBIS_PREC_MASK | IBIS_FORCING_OBSERVED_LW)) != 0 || prec > 2)
     return fail(IBIS_E_FLAGS, "ibis_forcing_diurnal: unknown flag bits or precision");
   if (ncell == 0) return IBIS_OK;
   const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
   hipStream_t s = (hipStream_t)hip_stream;
-  if (prec == IBIS_PREC_F64)
-    hipLaunchKernelGGL(forcing_diurnal_kernel<double>, grid, block, 0, s,
-                       (double*)state->planes, state->pitch, (const double*)daily->planes,
-                       daily->pitch, ncell, lat_deg, time, (double)jday, dtime, seed, cell0,
-                       step);
-  else
-    hipLaunchKernelGGL(forcing_diurnal_kernel<float>, grid, block, 0, s, (float*)state->planes,
-                       state->pitch, (const float*)daily->planes, daily->pitch, ncell, lat_deg,
-                       time, (double)jday, dtime, seed, cell0, step);
+  const bool obs = (flags & IBIS_FORCING_OBSERVED_LW) != 0;
+  if (prec == IBIS_PREC_F64) {
+    auto k = obs ? forcing_diurnal_kernel<double, true> : forcing_diurnal_kernel<double, false>;
+    hipLaunchKernelGGL(k, grid, block, 0, s, (double*)state->planes, state->pitch,
+                       (const double*)daily->planes, daily->pitch, ncell, lat_deg, time,
+                       (double)jday, dtime, seed, cell0, step);
+  } else {
+    auto k = obs ? forcing_diurnal_kernel<float, true> : forcing_diurnal_kernel<float, false>;
+    hipLaunchKernelGGL(k, grid, block, 0, s, (float*)state->planes, state->pitch,
+                       (const float*)daily->planes, daily->pitch, ncell, lat_deg, time,
+                       (double)jday, dtime, seed, cell0, step);
+  }
   return launch_error("ibis_forcing_diurnal");
+}
+
+int ibis_gswp3_days(const float* obs, int64_t obs_pitch, int ndays, int jday0,
+                    const double* lat_deg, int64_t ncell, double dtime, uint64_t seed,
+                    int64_t cell0, int64_t day0, const IbisArena* daily, int flags,
+                    void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0 || cell0 < 0 || day0 < 0 || ndays < 0 || ndays > 65535 || !(dtime > 0.0))
+    return fail(IBIS_E_ARG,
+                "ibis_gswp3_days: negative size, cell or day, ndays > 65535, or dtime <= 0");
+  if (jday0 < 1 || jday0 + ndays - 1 > 366)
+    return fail(IBIS_E_ARG, "ibis_gswp3_days: julian days %d..%d outside 1..366", jday0,
+                jday0 + ndays - 1);
+  if (!(86400.0 / dtime < 1e7))
+    return fail(IBIS_E_ARG, "ibis_gswp3_days: dtime %g too small", dtime);
+  if (check_arena(daily, ncell, "ibis_gswp3_days: null daily weather or pitch < ncell"))
+    return IBIS_E_ARG;
+  if (ncell > 0 && ndays > 0 && (!obs || obs_pitch < ncell || !lat_deg))
+    return fail(IBIS_E_ARG, "ibis_gswp3_days: null observations or latitude, or pitch < ncell");
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_gswp3_days: unknown flag bits or precision");
+  if (ncell == 0 || ndays == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock),
+                  (unsigned)((ndays + kDayChunk - 1) / kDayChunk));
+  const dim3 block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(gswp3_days_kernel<double>, grid, block, 0, s, obs, obs_pitch, ndays,
+                       jday0, lat_deg, ncell, dtime, seed, cell0, day0, (double*)daily->planes,
+                       daily->pitch);
+  else
+    hipLaunchKernelGGL(gswp3_days_kernel<float>, grid, block, 0, s, obs, obs_pitch, ndays,
+                       jday0, lat_deg, ncell, dtime, seed, cell0, day0, (float*)daily->planes,
+                       daily->pitch);
+  return launch_error("ibis_gswp3_days");
 }
 
 int ibis_accumulate(const IbisArena* state, const IbisArena* vegx, const IbisArena* acc,

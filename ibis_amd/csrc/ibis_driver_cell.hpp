@@ -33,6 +33,8 @@ IBIS_HD IBIS_INLINE double uniform01(uint64_t seed, uint64_t stream, uint64_t ce
 constexpr uint64_t kStreamTmean = 20, kStreamRh = 21, kStreamCloud = 22;
 constexpr uint64_t kStreamWind1 = 23, kStreamWind2 = 24, kStreamPwet = 25, kStreamPamt = 26;
 constexpr uint64_t kStreamDiurnalWind = 27;
+// the precipitation event of a GSWP3 day (per cell-day: hour argument = day index)
+constexpr uint64_t kStreamEventLen = 28, kStreamEventStart = 29;
 
 constexpr double kPi7 = 3.1415927;            // 7-digit pi of the benchmark generator
 constexpr double kPi = 3.141592653589793;      // weather.py's _PI (jnp.pi)
@@ -190,6 +192,93 @@ IBIS_HD IBIS_INLINE void diurnal_forcing(Forcing& o, double& psurf_out,
   o.raina = (in_event && warm) ? rate : 0.0;
   o.snowa = (in_event && !warm) ? rate : 0.0;
   psurf_out = psurf;
+}
+
+// ---- GSWP3 daily observations -> daily weather (gswp3_forcing.py:168-269) ----------------
+
+// Length, start and end (s) of the day's precipitation event (weather.py:395-441): a random
+// length of at least 4 h, extended so that no step carries more than 15 mm, placed at a
+// random whole step inside the day.
+IBIS_HD IBIS_INLINE void precip_event(double precip, double dtime, double u_len,
+                                      double u_start, double& plens, double& startp,
+                                      double& endp) {
+  const int64_t steps = (int64_t)(86400.0 / dtime);
+  const int64_t shortest = 1 + (int64_t)((4.0 * 3600.0 - 1.0) / dtime);
+  const int64_t day = (int64_t)(24.0 * 3600.0 / dtime);
+  const int64_t longest = day > shortest ? day : shortest;
+  int64_t n = shortest + (int64_t)(u_len * (double)(longest - shortest + 1));
+  n = n < shortest ? shortest : (n > longest ? longest : n);
+  const int64_t need = (int64_t)ceil(precip / 15.0);
+  n = n > need ? n : need;
+  n = n < shortest ? shortest : (n > longest ? longest : n);
+  const int64_t last = steps - n > 0 ? steps - n : 0;
+  int64_t first = (int64_t)(u_start * (double)(last + 1));
+  first = first < last ? first : last;
+  plens = dtime * (double)n;
+  startp = dtime * (double)first;
+  endp = startp + plens;
+}
+
+// Day terms of the solar geometry: sin, cos and tan of the Hartmann declination and the
+// effective solar constant of day `jday` (the same for every cell of the day).
+struct SunDay {
+  double sd, cd, td, sw;
+};
+IBIS_HD IBIS_INLINE SunDay sun_day(double jday) {
+  const double decl = declination(jday, kPi);
+  return SunDay{sin(decl), cos(decl), tan(decl), solar_constant(jday, kPi)};
+}
+
+// Cell terms: sin, cos and tan of the latitude, from degrees as np.deg2rad does.
+struct SunLat {
+  double sl, cl, tl;
+};
+IBIS_HD IBIS_INLINE SunLat sun_lat(double lat_deg) {
+  const double r = lat_deg * (kPi / 180.0);
+  return SunLat{sin(r), cos(r), tan(r)};
+}
+
+// 24-hour mean of max(0, cos zenith), from the sunset hour angle (gswp3_forcing.py:85-116).
+IBIS_HD IBIS_INLINE double daily_mean_coszen(const SunLat& L, const SunDay& D) {
+  double ch = -L.tl * D.td;
+  ch = ch < -1.0 ? -1.0 : (ch > 1.0 ? 1.0 : ch);
+  const double hs = acos(ch);
+  const double m = (L.sl * D.sd * hs + L.cl * D.cd * sin(hs)) / kPi;
+  return m > 0.0 ? m : 0.0;
+}
+
+// Cloud fraction that makes the generator's transmissivity 0.251 + 0.509 (1 - cloud) give the
+// observed daily mean shortwave; 0.5 where the mean cos zenith is below 0.01
+// (gswp3_forcing.py:119-165).
+IBIS_HD IBIS_INLINE double rsds_cloud(double rsds, const SunLat& L, const SunDay& D) {
+  const double mcz = daily_mean_coszen(L, D);
+  if (mcz < 0.01) return 0.5;
+  double trans = rsds / (D.sw * mcz);
+  trans = trans < 0.0 ? 0.0 : (trans > 1.0 ? 1.0 : trans);
+  const double c = 1.0 - (trans - 0.251) / 0.509;
+  return c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
+}
+
+// One cell-day: obs in the order tas, tasmax, tasmin, pr, rsds, rlds, huss, wind, ps; w in
+// the IBIS_NDAILY_OBS order of ibis_forcing_diurnal.  A missing observation reads as 0.  The
+// event length sees the precipitation as stored (type S), as the reference's float32 array.
+template <class S>
+IBIS_HD IBIS_INLINE void gswp3_day(double w[IBIS_NDAILY_OBS], const double obs[IBIS_NGSWP3],
+                                   const SunLat& L, const SunDay& D, double dtime, double u_len,
+                                   double u_start) {
+  double o[IBIS_NGSWP3];
+  for (int v = 0; v < IBIS_NGSWP3; ++v) o[v] = obs[v] != obs[v] ? 0.0 : obs[v];
+  const double precip = (double)S(o[3] * 86400.0);
+  w[0] = o[0];
+  w[1] = o[1];
+  w[2] = o[2];
+  w[3] = o[6];
+  w[4] = o[7];
+  w[5] = precip;
+  w[6] = rsds_cloud(o[4], L, D);
+  w[7] = o[8];
+  precip_event(precip, dtime, u_len, u_start, w[8], w[9], w[10]);
+  w[IBIS_DAILY_RLDS] = o[5];
 }
 
 // ---- per-step accumulators (loops_gswp3.py:161-302) ------------------------------------

@@ -176,6 +176,10 @@ extern "C" int ibis_hostcheck_forcing_diurnal(void* state, int64_t pitch, const 
     ibis::Forcing f;
     double ps;
     ibis::diurnal_forcing(f, ps, dw, lat[i], time, (double)jday, dtime, rn);
+    if (flags & IBIS_FORCING_OBSERVED_LW) {
+      const int64_t k = (int64_t)IBIS_DAILY_RLDS * dpitch + i;
+      f.fira = f64 ? ((const double*)daily)[k] : (double)((const float*)daily)[k];
+    }
     if (f64) {
       ibis::put_forcing((double*)state, pitch, i, f);
       ((double*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = ps;
@@ -184,6 +188,33 @@ extern "C" int ibis_hostcheck_forcing_diurnal(void* state, int64_t pitch, const 
       ((float*)state)[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = (float)ps;
     }
   }
+  return 0;
+}
+
+// ibis_gswp3_days on the host: obs float32 (IBIS_NGSWP3 x ndays planes of `opitch`), daily
+// in the precision's storage type (ndays blocks of IBIS_NDAILY_OBS planes of `dpitch`).
+extern "C" int ibis_hostcheck_gswp3_days(const float* obs, int64_t opitch, int ndays, int jday0,
+                                         const double* lat, int64_t ncell, double dtime,
+                                         uint64_t seed, int64_t cell0, int64_t day0,
+                                         void* daily, int64_t dpitch, int flags) {
+  const bool f64 = (flags & IBIS_PREC_MASK) == IBIS_PREC_F64;
+  for (int d = 0; d < ndays; ++d)
+    for (int64_t i = 0; i < ncell; ++i) {
+      double o[IBIS_NGSWP3], w[IBIS_NDAILY_OBS];
+      for (int v = 0; v < IBIS_NGSWP3; ++v) o[v] = (double)obs[((int64_t)v * ndays + d) * opitch + i];
+      const uint64_t c = (uint64_t)(cell0 + i), day = (uint64_t)(day0 + d);
+      const double ul = ibis::uniform01(seed, ibis::kStreamEventLen, c, day);
+      const double us = ibis::uniform01(seed, ibis::kStreamEventStart, c, day);
+      const ibis::SunLat L = ibis::sun_lat(lat[i]);
+      const ibis::SunDay D = ibis::sun_day((double)(jday0 + d));
+      if (f64) ibis::gswp3_day<double>(w, o, L, D, dtime, ul, us);
+      else ibis::gswp3_day<float>(w, o, L, D, dtime, ul, us);
+      for (int j = 0; j < IBIS_NDAILY_OBS; ++j) {
+        const int64_t k = ((int64_t)d * IBIS_NDAILY_OBS + j) * dpitch + i;
+        if (f64) ((double*)daily)[k] = w[j];
+        else ((float*)daily)[k] = (float)w[j];
+      }
+    }
   return 0;
 }
 

@@ -306,16 +306,24 @@ def _daily_arena(w: dict, pitch, dtype, device):
     return torch.as_tensor(a, device=device).to(dtype)
 
 
-def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR.Accumulators,
+def run_days(state: IBISState, day: DayState, weather, acc: DR.Accumulators,
              consts, params, days: int, p: IbisDayParams, dtime: float = 3600.0,
              precision=None, day0: int = 0, stream=None, on_day=None):
     """``days`` days of the production day loop: day begin, 24 hours (diurnal forcing from the
     day's weather, step, accumulators), day end.  Weather for every day is generated and
-    copied before the loop, so the loop itself only enqueues launches."""
+    copied before the loop, so the loop itself only enqueues launches.  ``weather`` is a
+    ``SyntheticWeather`` or a GSWP3 month (``ibis_amd.gswp3.MonthWeather``, days counted
+    from the month's first), whose hours take fira from the observed longwave
+    (loops_gswp3.py:131-148)."""
     st = DR._stream(state, stream)
     nstep = int(round(86400.0 / dtime))
-    wdays = [_daily_arena(weather.day(d), state.pitch, state.dtype, state.device)
-             for d in range(day0, day0 + days)]
+    if hasattr(weather, "device_days"):          # a GSWP3 month already on the device
+        wdays = weather.device_days(day0, days, state)
+    else:
+        wdays = [_daily_arena(weather.day(d), state.pitch, state.dtype, state.device)
+                 for d in range(day0, day0 + days)]
+    fflags = DR._code(state) | (DR.FORCING_OBSERVED_LW if getattr(weather, "observed_lw", False)
+                                else 0)
     cparams = S.ibis_params(state, consts, params)
     lat = torch.as_tensor(weather.lat, device=state.device)
     lib = DR._lib_driver()
@@ -332,7 +340,7 @@ def run_days(state: IBISState, day: DayState, weather: SyntheticWeather, acc: DR
                                           C.c_int64(state.n), C.c_double(h * dtime),
                                           C.c_int(weather.jday0 + d), C.c_double(dtime),
                                           C.c_uint64(weather.seed), C.c_int64(0),
-                                          C.c_int64(d * nstep + h), C.c_int(DR._code(state)),
+                                          C.c_int64(d * nstep + h), C.c_int(fflags),
                                           C.c_void_p(st.cuda_stream))
             _lib.check(rc, "ibis_forcing_diurnal")
             S.timestep_(state, consts, params, dtime, precision=precision, stream=st,

@@ -38,6 +38,8 @@ NACC = len(ACC_NAMES)
 NVEGX = 26
 DAILY_NAMES = ("td", "tmax", "tmin", "qd", "ud", "precip", "cloud", "psurf", "plens",
                "startp", "endp")
+#: ibis_forcing_diurnal flag: fira from the daily plane IBIS_DAILY_RLDS (observed longwave)
+FORCING_OBSERVED_LW = 0x100
 
 
 class IbisAccParams(C.Structure):
@@ -92,6 +94,10 @@ def _lib_driver():
         lib.ibis_count_nonfinite.argtypes = [C.POINTER(abi.IbisArena), C.c_int64, C.c_int,
                                              C.c_void_p, C.c_void_p]
         lib.ibis_count_nonfinite.restype = C.c_int
+        lib.ibis_gswp3_days.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p,
+                                        C.c_int64, C.c_double, C.c_uint64, C.c_int64, C.c_int64,
+                                        C.POINTER(abi.IbisArena), C.c_int, C.c_void_p]
+        lib.ibis_gswp3_days.restype = C.c_int
         lib._driver_typed = True
     return lib
 

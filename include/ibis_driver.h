@@ -49,6 +49,36 @@ int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
                          double dtime, uint64_t seed, int64_t cell0, int64_t step, int flags,
                          void* hip_stream);
 
+/* Observed longwave: with IBIS_FORCING_OBSERVED_LW in the flags of ibis_forcing_diurnal,
+ * `daily` holds one more plane, IBIS_DAILY_RLDS (the day's observed downwelling longwave,
+ * W m-2), and fira takes it in place of the Idso longwave, as the GSWP3 month driver does
+ * (python-port/ibis/timestepping/loops_gswp3.py:131-148). */
+#define IBIS_DAILY_RLDS 11
+#define IBIS_NDAILY_OBS 12
+#define IBIS_FORCING_OBSERVED_LW 0x100
+
+/* GSWP3 daily observations on the land cells, in the reader's variable order and native
+ * units (python-port/ibis/io/gswp3_reader.py:44-45): tas, tasmax, tasmin (K), pr
+ * (kg m-2 s-1), rsds, rlds (W m-2), huss (kg/kg), wind (m/s), ps (Pa). */
+#define IBIS_NGSWP3 9
+
+/* Convert `ndays` consecutive days of GSWP3 observations into the daily weather of
+ * ibis_forcing_diurnal with the observed longwave (python-port/ibis/io/gswp3_forcing.py:
+ * 168-269): a missing value (NaN) reads as 0; precipitation x 86400 to mm/day; cloud from
+ * the daily mean shortwave by inverting the transmissivity 0.251 + 0.509 (1 - cloud) against
+ * the 24-hour mean cosine of the zenith angle (0.5 where that mean is below 0.01); and the
+ * day's precipitation event (python-port/ibis/physics/weather.py:395-441), whose two uniform
+ * draws come from the counter-based hash of (seed, cell0 + i, day0 + d) in place of the
+ * reference's jax.random key.  Day d has julian day jday0 + d (days of one month).
+ * obs: float32 device memory, IBIS_NGSWP3 x ndays planes of `obs_pitch` (plane
+ * v * ndays + d holds variable v of day d).  lat_deg: ncell doubles.  daily: ndays blocks of
+ * IBIS_NDAILY_OBS planes (block d starts at plane d * IBIS_NDAILY_OBS), in the storage type
+ * of the precision bits of `flags` (float64 for IBIS_PREC_F64, float32 otherwise). */
+int ibis_gswp3_days(const float* obs, int64_t obs_pitch, int ndays, int jday0,
+                    const double* lat_deg, int64_t ncell, double dtime, uint64_t seed,
+                    int64_t cell0, int64_t day0, const IbisArena* daily, int flags,
+                    void* hip_stream);
+
 /* Per-step accumulators: IBIS_NACC sums, each with a compensation plane (Kahan), so that
  * an FP32 accumulator over a year of hours keeps the accuracy of an FP64 one. */
 enum IbisAcc {

@@ -1,0 +1,106 @@
+"""GSWP3 forcing on the GPU: ``ibis_gswp3_days`` against the oracle and the host build, the
+observed-longwave hour (loops_gswp3.py:131-148), and a GSWP3 month through the day loop."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from ibis_amd import abi, daily as DY, driver as DR, gswp3 as G, synth
+from ibis_amd.params import default_sim_params, get_constants, root_fractions
+from ibis_amd.state import IBISState
+from oracle import gswp3 as OG
+
+from test_gswp3 import _host_days, _obs, oracle_days
+
+pytestmark = pytest.mark.gpu
+
+
+def _month(obs, year, month):
+    m = dict(obs)
+    m.update(ndays=obs["tas"].shape[0], year=year, month=month)
+    return m
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_days_match_the_oracle_and_the_host_build(prec):
+    n, nd = 5000, 29
+    print(f"\ngswp3 days: {prec} n={n} days={nd}", flush=True)
+    lat = np.linspace(-89.75, 89.75, n)
+    obs = _obs(n, nd, 11)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    code = abi.PREC_F64 if prec == "f64" else abi.PREC_MIXED
+    w = G.MonthWeather(_month(obs, 1980, 1), lat, seed=13, day0=31, dtype=dt)
+    torch.cuda.synchronize()
+    got = w.days.cpu().numpy()
+    assert w.jday0 == 32 and got.shape == (nd, G.NDAILY_OBS, n)
+    store = np.float64 if prec == "f64" else np.float32
+    ref = oracle_days(obs, lat, 32, 13, 31, store)
+    host = _host_days(obs, lat, 32, 13, 31, code)
+    for j, k in enumerate(OG.DAILY_OBS):
+        if prec == "f64":
+            np.testing.assert_allclose(got[:, j], ref[:, j], rtol=1e-12, atol=1e-300, err_msg=k)
+        else:
+            np.testing.assert_allclose(got[:, j], ref[:, j].astype(np.float32), rtol=2.5e-7,
+                                       atol=0, err_msg=k)
+            np.testing.assert_allclose(got[:, j], host[:, j], rtol=2.5e-7, atol=0, err_msg=k)
+    for j in (0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11):      # no transcendental on the way: exact
+        np.testing.assert_array_equal(got[:, j], host[:, j], err_msg=OG.DAILY_OBS[j])
+
+
+def test_observed_longwave_replaces_the_idso_longwave():
+    n = 3000
+    f, slot, lat = synth.make_cells(n, 31)
+    st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda",
+                              dtype=torch.float32)
+    obs = _obs(n, 3, 12)
+    w = G.MonthWeather(_month(obs, 1979, 6), np.asarray(lat), seed=3, pitch=st.pitch)
+    lib = DR._lib_driver()
+    latd = torch.as_tensor(np.asarray(lat, np.float64), device="cuda")
+    outs = []
+    for flags in (abi.PREC_MIXED, abi.PREC_MIXED | G.FORCING_OBSERVED_LW):
+        s = st.clone()
+        a = abi.IbisArena(s.arena.data_ptr(), s.pitch)
+        blk = abi.IbisArena(w.days[1].data_ptr(), s.pitch)
+        rc = lib.ibis_forcing_diurnal(C.byref(a), C.byref(blk), C.c_void_p(latd.data_ptr()),
+                                      C.c_int64(n), C.c_double(13 * 3600.0), C.c_int(w.jday0 + 1),
+                                      C.c_double(3600.0), C.c_uint64(3), C.c_int64(0),
+                                      C.c_int64(37), C.c_int(flags), None)
+        assert rc == 0
+        outs.append(s)
+    torch.cuda.synchronize()
+    plain, obsd = outs
+    np.testing.assert_array_equal(obsd.atm.fira[:n].cpu().numpy(), obs["rlds"][1])
+    assert not torch.equal(plain.atm.fira[:n], obsd.atm.fira[:n])
+    for k in ("ta", "qa", "ua", "coszen", "raina", "snowa", "psurf"):
+        assert torch.equal(getattr(plain.atm, k), getattr(obsd.atm, k)), k
+    assert torch.equal(plain.atm.solad, obsd.atm.solad)
+
+
+def test_a_gswp3_month_through_the_day_loop():
+    """Two days of the production day loop on a GSWP3 month: fira is the day's observed
+    longwave and psurf its observed pressure after every hour, and everything stays finite."""
+    n = 2048
+    f, slot, lat = synth.make_cells(n, 23)
+    st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda",
+                              dtype=torch.float32)
+    month = G.synthetic_month(n, 1985, 5, seed=4)
+    w = G.MonthWeather(month, np.asarray(lat), seed=9, pitch=st.pitch)
+    rng = np.random.default_rng(3)
+    vals = dict(tempu=np.full(n, 0.5), templ=np.full(n, 0.5), agddu=np.full(n, 80.0),
+                agddl=np.full(n, 120.0), dropu=np.ones(n), dropls=np.ones(n), dropl3=np.ones(n),
+                dropl4=np.ones(n), plai=rng.uniform(0.5, 5, (n, 12)), tc=rng.uniform(-20, 15, n),
+                a10td=np.asarray(f["soil.tsoi"])[:, 0], csoipas=np.full(n, 8.0),
+                totcmic=np.full(n, 0.1))
+    day = DY.DayState(st, vals)
+    acc = DR.Accumulators(st)
+    p = DY.day_params(synth.HSOI)
+    DY.run_days(st, day, w, acc, get_constants(), default_sim_params(), 2, p, day0=3)
+    torch.cuda.synchronize()
+    assert p.jday == w.jday0 + 4 == 152 + 4
+    np.testing.assert_array_equal(st.atm.fira[:n].cpu().numpy(), month["rlds"][4])
+    np.testing.assert_array_equal(st.atm.psurf[:n].cpu().numpy(), month["ps"][4])
+    assert torch.isfinite(st.arena[:, :n]).all()
+    assert torch.isfinite(acc.acc).all()
+    with pytest.raises(ValueError):
+        w.device_days(28, 3, st)
