@@ -43,6 +43,20 @@ __global__ __launch_bounds__(kDriverBlock) void forcing_diurnal_kernel(
   a[(int64_t)IBIS_P_ATM_PSURF * pitch + i] = S(ps);
 }
 
+template <class S>
+__global__ __launch_bounds__(kDriverBlock) void forcing_observed_sw_kernel(
+    S* a, int64_t pitch, const S* sw, const uint8_t* use, int64_t ncell, double jday) {
+  const int64_t i = (int64_t)blockIdx.x * kDriverBlock + threadIdx.x;
+  if (i >= ncell || use[i] == 0) return;
+  Forcing f;
+  observed_shortwave(f, (double)sw[i], (double)a[(int64_t)IBIS_P_ATM_COSZEN * pitch + i], jday);
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    a[(int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i] = S(f.solad[b]);
+    a[(int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i] = S(f.solai[b]);
+  }
+}
+
 // One thread per cell and blockIdx.y = a chunk of kDayChunk days: the day terms of the solar
 // geometry are computed once per block into LDS and the latitude terms once per thread, so a
 // cell-day costs two transcendentals (the sunset hour angle and its sine) besides its
@@ -186,6 +200,32 @@ int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
                        (double)jday, dtime, seed, cell0, step);
   }
   return launch_error("ibis_forcing_diurnal");
+}
+
+int ibis_forcing_observed_sw(const IbisArena* state, const void* sw, const uint8_t* use,
+                             int64_t ncell, int jday, int flags, void* hip_stream) {
+  using namespace ibis;
+  err_buf()[0] = 0;
+  if (ncell < 0) return fail(IBIS_E_ARG, "ibis_forcing_observed_sw: negative size");
+  if (check_arena(state, ncell, "ibis_forcing_observed_sw: null state or pitch < ncell"))
+    return IBIS_E_ARG;
+  if (ncell > 0 && (!sw || !use))
+    return fail(IBIS_E_ARG, "ibis_forcing_observed_sw: null shortwave or mask");
+  const int prec = flags & IBIS_PREC_MASK;
+  if ((flags & ~IBIS_PREC_MASK) != 0 || prec > 2)
+    return fail(IBIS_E_FLAGS, "ibis_forcing_observed_sw: unknown flag bits or precision");
+  if (ncell == 0) return IBIS_OK;
+  const dim3 grid((unsigned)((ncell + kDriverBlock - 1) / kDriverBlock)), block(kDriverBlock);
+  hipStream_t s = (hipStream_t)hip_stream;
+  if (prec == IBIS_PREC_F64)
+    hipLaunchKernelGGL(forcing_observed_sw_kernel<double>, grid, block, 0, s,
+                       (double*)state->planes, state->pitch, (const double*)sw, use, ncell,
+                       (double)jday);
+  else
+    hipLaunchKernelGGL(forcing_observed_sw_kernel<float>, grid, block, 0, s,
+                       (float*)state->planes, state->pitch, (const float*)sw, use, ncell,
+                       (double)jday);
+  return launch_error("ibis_forcing_observed_sw");
 }
 
 int ibis_gswp3_days(const float* obs, int64_t obs_pitch, int ndays, int jday0,

@@ -194,6 +194,22 @@ IBIS_HD IBIS_INLINE void diurnal_forcing(Forcing& o, double& psurf_out,
   psurf_out = psurf;
 }
 
+// The observed hourly shortwave `sw` split as weather.py:458-480 does, into the band arrays
+// of `f`, given the step's cos zenith.
+IBIS_HD IBIS_INLINE void observed_shortwave(Forcing& f, double sw, double coszen, double jday) {
+  double den = solar_constant(jday, kPi) * coszen;
+  den = den > 1.0 ? den : 1.0;
+  double kt = sw / den;
+  kt = kt < 0.0 ? 0.0 : (kt > 1.0 ? 1.0 : kt);
+  const double fd = spitters_fd(kt, coszen);
+  const double s = sw > 0.0 ? sw : 0.0;
+  const double band[2] = {0.46, 0.54};
+  for (int b = 0; b < 2; ++b) {
+    f.solad[b] = s * band[b] * (1.0 - fd);
+    f.solai[b] = s * band[b] * fd;
+  }
+}
+
 // ---- GSWP3 daily observations -> daily weather (gswp3_forcing.py:168-269) ----------------
 
 // Length, start and end (s) of the day's precipitation event (weather.py:395-441): a random

@@ -191,6 +191,32 @@ extern "C" int ibis_hostcheck_forcing_diurnal(void* state, int64_t pitch, const 
   return 0;
 }
 
+extern "C" int ibis_hostcheck_forcing_observed_sw(void* state, int64_t pitch, const void* sw,
+                                                  const uint8_t* use, int64_t ncell, int jday,
+                                                  int flags) {
+  const bool f64 = (flags & IBIS_PREC_MASK) == IBIS_PREC_F64;
+  for (int64_t i = 0; i < ncell; ++i) {
+    if (!use[i]) continue;
+    const int64_t kz = (int64_t)IBIS_P_ATM_COSZEN * pitch + i;
+    const double cz = f64 ? ((double*)state)[kz] : (double)((float*)state)[kz];
+    const double s = f64 ? ((const double*)sw)[i] : (double)((const float*)sw)[i];
+    ibis::Forcing f;
+    ibis::observed_shortwave(f, s, cz, (double)jday);
+    for (int b = 0; b < 2; ++b) {
+      const int64_t kd = (int64_t)(IBIS_P_ATM_SOLAD + b) * pitch + i;
+      const int64_t ki = (int64_t)(IBIS_P_ATM_SOLAI + b) * pitch + i;
+      if (f64) {
+        ((double*)state)[kd] = f.solad[b];
+        ((double*)state)[ki] = f.solai[b];
+      } else {
+        ((float*)state)[kd] = (float)f.solad[b];
+        ((float*)state)[ki] = (float)f.solai[b];
+      }
+    }
+  }
+  return 0;
+}
+
 // ibis_gswp3_days on the host: obs float32 (IBIS_NGSWP3 x ndays planes of `opitch`), daily
 // in the precision's storage type (ndays blocks of IBIS_NDAILY_OBS planes of `dpitch`).
 extern "C" int ibis_hostcheck_gswp3_days(const float* obs, int64_t opitch, int ndays, int jday0,

@@ -324,6 +324,10 @@ def run_days(state: IBISState, day: DayState, weather, acc: DR.Accumulators,
                  for d in range(day0, day0 + days)]
     fflags = DR._code(state) | (DR.FORCING_OBSERVED_LW if getattr(weather, "observed_lw", False)
                                 else 0)
+    sw_obs, sw_use = getattr(weather, "sw_hourly", None), getattr(weather, "sw_use", None)
+    if sw_obs is not None and (sw_obs.dtype != state.dtype or sw_obs.shape[2] != state.pitch
+                               or sw_obs.shape[1] != nstep):
+        raise ValueError("observed shortwave: dtype, pitch or steps per day differ from the run")
     cparams = S.ibis_params(state, consts, params)
     lat = torch.as_tensor(weather.lat, device=state.device)
     lib = DR._lib_driver()
@@ -343,6 +347,13 @@ def run_days(state: IBISState, day: DayState, weather, acc: DR.Accumulators,
                                           C.c_int64(d * nstep + h), C.c_int(fflags),
                                           C.c_void_p(st.cuda_stream))
             _lib.check(rc, "ibis_forcing_diurnal")
+            if sw_obs is not None:                # the tower's hourly shortwave (site runs)
+                rc = lib.ibis_forcing_observed_sw(C.byref(a), C.c_void_p(sw_obs[d, h].data_ptr()),
+                                                  C.c_void_p(sw_use.data_ptr()),
+                                                  C.c_int64(state.n), C.c_int(weather.jday0 + d),
+                                                  C.c_int(DR._code(state)),
+                                                  C.c_void_p(st.cuda_stream))
+                _lib.check(rc, "ibis_forcing_observed_sw")
             S.timestep_(state, consts, params, dtime, precision=precision, stream=st,
                         cparams=cparams, cext=cext)
             acc.add(state, stream=st)

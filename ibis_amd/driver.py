@@ -98,6 +98,9 @@ def _lib_driver():
                                         C.c_int64, C.c_double, C.c_uint64, C.c_int64, C.c_int64,
                                         C.POINTER(abi.IbisArena), C.c_int, C.c_void_p]
         lib.ibis_gswp3_days.restype = C.c_int
+        lib.ibis_forcing_observed_sw.argtypes = [C.POINTER(abi.IbisArena), C.c_void_p, C.c_void_p,
+                                                 C.c_int64, C.c_int, C.c_int, C.c_void_p]
+        lib.ibis_forcing_observed_sw.restype = C.c_int
         lib._driver_typed = True
     return lib
 

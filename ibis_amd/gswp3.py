@@ -20,7 +20,9 @@ as the netCDF4 library does by default.
 ``ibis_gswp3_days`` (include/ibis_driver.h) into the daily weather of the diurnal generator
 with the observed longwave (gswp3_forcing.py:168-269, loops_gswp3.py:650-686); it is the
 ``weather`` argument of ``ibis_amd.daily.run_days`` for the month, whose hourly forcing then
-takes fira from the observed rlds (loops_gswp3.py:131-148).
+takes fira from the observed rlds (loops_gswp3.py:131-148), and, where the month carries the
+tower's hourly shortwave (``rsds_hourly``, ``rsds_hourly_ok``: site runs), solad and solai
+from it (``ibis_forcing_observed_sw``, loops_gswp3.py:126-129).
 """
 from __future__ import annotations
 
@@ -267,6 +269,24 @@ class MonthWeather:
         self._arena = abi.IbisArena(self.days.data_ptr(), pitch)
         self.n, self.cell0, self.day0 = n, int(cell0), int(day0)
         self.convert()
+        # the tower's observed hourly shortwave where the month carries one (site runs,
+        # loops_gswp3.py:677-686): (ndays, steps per day, npoi) W m-2 and an (npoi,) mask
+        self.sw_hourly = self.sw_use = None
+        if month.get("rsds_hourly") is not None:
+            nsteps = int(86400.0 / self.dtime)
+            sw = np.asarray(month["rsds_hourly"], np.float64)
+            if sw.shape != (self.ndays, nsteps, n):
+                raise ValueError(f"rsds_hourly: shape {sw.shape}, expected {(self.ndays, nsteps, n)}")
+            use = month.get("rsds_hourly_ok")
+            use = np.ones(n, bool) if use is None else np.asarray(use, bool)
+            if use.shape != (n,):
+                raise ValueError(f"rsds_hourly_ok: shape {use.shape}, expected {(n,)}")
+            pad = np.zeros((self.ndays, nsteps, pitch))
+            pad[:, :, :n] = sw
+            self.sw_hourly = torch.as_tensor(pad, device=device).to(dtype)
+            mask = np.zeros(pitch, np.uint8)
+            mask[:n] = use
+            self.sw_use = torch.as_tensor(mask, device=device)
 
     def convert(self, stream=None):
         """(Re)build ``self.days`` from the observations on the device: one launch."""

@@ -57,6 +57,18 @@ int ibis_forcing_diurnal(const IbisArena* state, const IbisArena* daily,
 #define IBIS_NDAILY_OBS 12
 #define IBIS_FORCING_OBSERVED_LW 0x100
 
+/* The tower's observed hourly shortwave in place of the generator's cycle (site runs of the
+ * month driver, python-port/ibis/timestepping/loops_gswp3.py:126-129, with
+ * python-port/ibis/physics/weather.py:458-480): where use[i] is nonzero, sw[i] (W m-2, the
+ * step's total) is split into direct and diffuse by the Spitters et al. (1986) relation on
+ * the clearness index sw / max(S0(jday) x coszen, 1), all diffuse with the sun down, and into
+ * the visible and near-infrared bands (0.46, 0.54); elsewhere the state's solad and solai are
+ * left as they are.  coszen is read from the state: call after ibis_forcing_diurnal of the
+ * same step.  sw: ncell values in the state's storage type (flags' precision bits); use:
+ * ncell bytes; both device memory. */
+int ibis_forcing_observed_sw(const IbisArena* state, const void* sw, const uint8_t* use,
+                             int64_t ncell, int jday, int flags, void* hip_stream);
+
 /* GSWP3 daily observations on the land cells, in the reader's variable order and native
  * units (python-port/ibis/io/gswp3_reader.py:44-45): tas, tasmax, tasmin (K), pr
  * (kg m-2 s-1), rsds, rlds (W m-2), huss (kg/kg), wind (m/s), ps (Pa). */

@@ -104,3 +104,66 @@ def test_a_gswp3_month_through_the_day_loop():
     assert torch.isfinite(acc.acc).all()
     with pytest.raises(ValueError):
         w.device_days(28, 3, st)
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_observed_shortwave_matches_the_oracle(prec):
+    from ibis_amd import layout as L
+    n = 5000
+    rng = np.random.default_rng(6)
+    dt = np.float64 if prec == "f64" else np.float32
+    code = abi.PREC_F64 if prec == "f64" else abi.PREC_MIXED
+    a = rng.random((L.NPLANE, n)).astype(dt)
+    cz = np.concatenate([np.zeros(300), rng.uniform(0.02, 1.0, n - 300)]).astype(dt)
+    a[L.PLANE_OFFSET["atm.coszen"]] = cz
+    sw = rng.uniform(-5.0, 1100.0, n).astype(dt)
+    use = (rng.random(n) < 0.7).astype(np.uint8)
+    od, oi = L.PLANE_OFFSET["atm.solad"], L.PLANE_OFFSET["atm.solai"]
+    ad, sd, ud = (torch.as_tensor(x, device="cuda") for x in (a, sw, use))
+    ar = abi.IbisArena(ad.data_ptr(), n)
+    rc = DR._lib_driver().ibis_forcing_observed_sw(C.byref(ar), C.c_void_p(sd.data_ptr()),
+                                                   C.c_void_p(ud.data_ptr()), C.c_int64(n),
+                                                   C.c_int(200), C.c_int(code), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = ad.cpu().numpy()
+    d, i = OG.observed_shortwave(sw.astype(np.float64), use, cz.astype(np.float64), 200,
+                                 a[od:od + 2].T.astype(np.float64), a[oi:oi + 2].T.astype(np.float64))
+    tol = dict(rtol=1e-12, atol=1e-300) if prec == "f64" else dict(rtol=2.5e-7, atol=0)
+    np.testing.assert_allclose(got[od:od + 2].T, d.astype(dt), **tol)
+    np.testing.assert_allclose(got[oi:oi + 2].T, i.astype(dt), **tol)
+    other = np.ones(L.NPLANE, bool)
+    other[[od, od + 1, oi, oi + 1]] = False
+    np.testing.assert_array_equal(got[other], a[other])
+
+
+def test_a_site_month_with_the_towers_hourly_shortwave():
+    """The day loop on a month that carries the tower's hourly shortwave: where the mask is
+    set, the last hour's solad + solai is its observed total; elsewhere the generator's."""
+    n = 1024
+    f, slot, lat = synth.make_cells(n, 24)
+    month = G.synthetic_month(n, 2001, 6, seed=5)
+    rng = np.random.default_rng(8)
+    month["rsds_hourly"] = rng.uniform(0.0, 900.0, (month["ndays"], 24, n)).astype(np.float32)
+    month["rsds_hourly_ok"] = rng.random(n) < 0.5
+    runs = []
+    for with_sw in (False, True):
+        st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda",
+                                  dtype=torch.float32)
+        m = month if with_sw else {k: v for k, v in month.items() if not k.startswith("rsds_h")}
+        w = G.MonthWeather(m, np.asarray(lat), seed=9, pitch=st.pitch)
+        day = DY.DayState(st, dict(plai=np.full((n, 12), 2.0), tc=np.full(n, 5.0),
+                                   a10td=np.asarray(f["soil.tsoi"])[:, 0]))
+        DY.run_days(st, day, w, DR.Accumulators(st), get_constants(), default_sim_params(), 1,
+                    DY.day_params(synth.HSOI), day0=2)
+        torch.cuda.synchronize()
+        assert torch.isfinite(st.arena[:, :n]).all()
+        runs.append(st)
+    plain, obs = runs
+    use = month["rsds_hourly_ok"]
+    tot = (obs.atm.solad[:n] + obs.atm.solai[:n]).sum(dim=1).cpu().numpy()
+    np.testing.assert_allclose(tot[use], month["rsds_hourly"][2, 23][use], rtol=1e-6, atol=1e-4)
+    assert torch.equal(plain.atm.coszen, obs.atm.coszen)
+    keep = torch.as_tensor(~use, device="cuda")
+    assert torch.equal(plain.atm.solad[:n][keep], obs.atm.solad[:n][keep])
+    assert torch.equal(plain.atm.solai[:n][keep], obs.atm.solai[:n][keep])

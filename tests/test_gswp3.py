@@ -230,3 +230,54 @@ def test_unreadable_variable_is_nan_and_hdf5_is_refused(gdir):
         G.GSWP3Reader(gdir, variables=("tas",)).read_month(1985, 0, np.array([0]), np.array([0]))
     with pytest.raises(FileNotFoundError):
         G.GSWP3Reader(os.path.join(gdir, "nowhere"))
+
+
+# ---- the tower's observed hourly shortwave -------------------------------------------------
+
+def test_observed_shortwave_pins():
+    n = 400
+    rng = np.random.default_rng(2)
+    cz = np.concatenate([np.zeros(50), rng.uniform(0.05, 1.0, n - 50)])
+    sw = rng.uniform(-5.0, 1100.0, n)
+    gen_d, gen_i = rng.random((n, 2)), rng.random((n, 2))
+    use = rng.random(n) < 0.8
+    d, i = OG.observed_shortwave(sw, use, cz, 172, gen_d, gen_i)
+    np.testing.assert_allclose((d + i).sum(axis=1)[use], np.maximum(sw, 0.0)[use], rtol=1e-14)
+    np.testing.assert_array_equal(d[~use], gen_d[~use])
+    np.testing.assert_array_equal(i[~use], gen_i[~use])
+    assert (d[:50][use[:50]] == 0.0).all()                          # sun down: all diffuse
+    kt = sw / np.maximum(OG.solar_constant(172) * cz, 1.0)
+    cloudy = use & (kt <= 0.22)
+    assert (d[cloudy] == 0.0).all()
+    r = 0.847 - 1.61 * cz + 1.04 * cz ** 2
+    clear = use & (cz > 0) & (kt > (1.47 - r) / 1.66) & (kt < 1.0) & (r >= 0) & (r <= 1)
+    assert clear.sum() > 10
+    np.testing.assert_allclose(i[clear].sum(axis=1), (sw * r)[clear], rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_F64, abi.PREC_MIXED])
+def test_observed_shortwave_host_build_matches_the_oracle(prec):
+    from ibis_amd import layout as L
+    n = 600
+    rng = np.random.default_rng(4)
+    dt = np.float64 if prec == abi.PREC_F64 else np.float32
+    a = rng.random((L.NPLANE, n)).astype(dt)
+    cz = np.concatenate([np.zeros(60), rng.uniform(0.02, 1.0, n - 60)]).astype(dt)
+    a[L.PLANE_OFFSET["atm.coszen"]] = cz
+    sw = rng.uniform(-5.0, 1100.0, n).astype(dt)
+    use = (rng.random(n) < 0.7).astype(np.uint8)
+    od, oi = L.PLANE_OFFSET["atm.solad"], L.PLANE_OFFSET["atm.solai"]
+    gen_d, gen_i = a[od:od + 2].T.astype(np.float64), a[oi:oi + 2].T.astype(np.float64)
+    before = a.copy()
+    rc = C.CDLL(HOSTCHECK).ibis_hostcheck_forcing_observed_sw(
+        a.ctypes.data_as(C.c_void_p), C.c_int64(n), sw.ctypes.data_as(C.c_void_p),
+        use.ctypes.data_as(C.c_void_p), C.c_int64(n), C.c_int(200), C.c_int(prec))
+    assert rc == 0
+    d, i = OG.observed_shortwave(sw.astype(np.float64), use, cz.astype(np.float64), 200,
+                                 gen_d, gen_i)
+    tol = dict(rtol=1e-13, atol=1e-300) if prec == abi.PREC_F64 else dict(rtol=2.5e-7, atol=0)
+    np.testing.assert_allclose(a[od:od + 2].T, d.astype(dt), **tol)
+    np.testing.assert_allclose(a[oi:oi + 2].T, i.astype(dt), **tol)
+    other = np.ones(L.NPLANE, bool)
+    other[[od, od + 1, oi, oi + 1]] = False
+    np.testing.assert_array_equal(a[other], before[other])
