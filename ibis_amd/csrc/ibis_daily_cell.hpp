@@ -334,6 +334,15 @@ IBIS_HD IBIS_INLINE void day_end_cell(const DayView<S>& v, S* vegx, int64_t vpit
   // annual heterotrophic and root respiration (loops_gswp3.py:577-585)
   v.d(IBIS_DAY_AYCO2MIC) = v.d(IBIS_DAY_AYCO2MIC) + tco2mic * S(12.0e-3 * 86400.0);
   v.d(IBIS_DAY_AYCO2ROOT) = v.d(IBIS_DAY_AYCO2ROOT) + v.d(IBIS_DAY_TCO2ROOT) * S(12.0e-3 * 86400.0);
+  // soil-water stress of the tree-establishment gate, growing-season days only
+  // (loops_gswp3.py:587-599)
+  if (p.stress) {
+    S* sa = (S*)p.stress;
+    const S grow = v.td() > S(p.tmelt + p.stress_accum_tmin) ? S(1) : S(0);
+    sa[v.i] = sa[v.i] + v.s(IBIS_P_SOIL_STRESSTU) * grow;
+    sa[v.dp + v.i] = sa[v.dp + v.i] + v.s(IBIS_P_SOIL_STRESSTL) * grow;
+    sa[2 * v.dp + v.i] = sa[2 * v.dp + v.i] + grow;
+  }
 }
 
 }  // namespace ibis

@@ -141,7 +141,8 @@ class IbisDayParams(C.Structure):
                    ("acclim_tau", C.c_double)] + [(n, C.c_double) for n in _SEN]
                 + [("sen_photoperiod_u", C.c_int32), ("sen_photoperiod_l", C.c_int32),
                    ("jday", C.c_int32), ("pad1", C.c_int32), ("seasonal", C.c_void_p),
-                   ("lat_rad", C.c_void_p)])
+                   ("lat_rad", C.c_void_p), ("stress_accum_tmin", C.c_double),
+                   ("stress", C.c_void_p)])
 
 
 def day_params(hsoi, pheno: PhenoParams | None = None, bgc: BGCParams | None = None,
@@ -184,6 +185,27 @@ def attach_seasonal(state: IBISState, p: IbisDayParams, lat_deg, acclim_x0: floa
     lat_t = torch.as_tensor(lat, device=state.device)
     p.seasonal, p.lat_rad = block.data_ptr(), lat_t.data_ptr()
     p._keep = (block, lat_t)
+    return block
+
+
+#: the soil-water stress accumulator (python-port/ibis/state/stress_accum.py:15-26), planes of
+#: a (3, pitch) block
+STRESS_FIELDS = ("aystressu", "aystressl", "nday")
+
+
+def attach_stress(state: IBISState, p: IbisDayParams, tmin_c: float = -100.0):
+    """Give ``state`` the stress accumulator (zero sums and day count, as
+    ``create_stress_accum``) as a (3, pitch) device block whose planes are the state's
+    ``stress.*`` leaves, and point ``p`` at it with the growing-season threshold
+    ``tmin_c`` (PhenoParams.stress_accum_tmin, deg C).  Returns the block."""
+    block = torch.zeros((3, state.pitch), dtype=state.dtype, device=state.device)
+    for j, k in enumerate(STRESS_FIELDS):
+        old = state.aux.get(f"stress.{k}")
+        if old is not None:
+            block[j].copy_(old)
+        state.aux[f"stress.{k}"] = block[j]
+    p.stress, p.stress_accum_tmin = block.data_ptr(), float(tmin_c)
+    p._keep_stress = block
     return block
 
 

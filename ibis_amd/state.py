@@ -21,7 +21,8 @@ _GROUPS = ("soil", "snow", "canopy", "atm", "vegetation", "hydrology", "diag", "
 #: optional per-cell leaves kept beside the arena (read by the optional StomataParams
 #: switches): the SeasonalState (python-port/ibis/state/seasonal.py:36-43) and the day's mean
 #: air temperature AtmState.td
-AUX_KEYS = ("seasonal.acclim_x", "seasonal.ssen_u", "seasonal.ssen_l", "atm.td")
+AUX_KEYS = ("seasonal.acclim_x", "seasonal.ssen_u", "seasonal.ssen_l", "atm.td", "stress.aystressu",
+            "stress.aystressl", "stress.nday")
 
 
 class _Group:

@@ -83,6 +83,14 @@ typedef struct IbisDayParams {
   int32_t jday, pad1;        /* day of year of this day, 1-based */
   void* seasonal;
   const double* lat_rad;
+  /* Soil-water stress accumulator of the tree-establishment gate (state/stress_accum.py:15-26;
+   * loops_gswp3.py:587-599).  With `stress` set, day end adds the canopies' stress factors
+   * soil.stresstu / stresstl (after the day's last hour) to the sums and 1 to the day count on
+   * growing-season days, td > tmelt + stress_accum_tmin (deg C; -100: every day).  stress: 3
+   * planes of the day arena's pitch (aystressu, aystressl, nday) in the state's element type,
+   * device memory. */
+  double stress_accum_tmin;
+  void* stress;
 } IbisDayParams;
 
 /* Day begin.  state: hourly-step arena (vegetation frac / lai / ztop / zbot written, fu fl

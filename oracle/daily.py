@@ -395,3 +395,14 @@ def day_end(day, sums, td, tsoi, wsoi, wisoi, sand, clay, hsoi, bgc: BGCParams, 
     out["ayco2mic"] = day["ayco2mic"] + tco2mic * (12.0e-3 * 86400.0)
     out["ayco2root"] = day["ayco2root"] + day["tco2root"] * (12.0e-3 * 86400.0)
     return out, tco2mic, tnmin
+
+
+def stress_accum(acc, td, stresstu, stresstl, tmelt, tmin):
+    """The soil-water stress accumulator at day end (loops_gswp3.py:587-599,
+    state/stress_accum.py:15-26): growing-season days (td > tmelt + tmin) add the canopies'
+    stress factors and one day.  Pinned by its closed form over consecutive days
+    (tests/test_daily.py)."""
+    grow = (np.asarray(td) > tmelt + tmin).astype(np.float64)
+    return dict(aystressu=acc["aystressu"] + stresstu * grow,
+                aystressl=acc["aystressl"] + stresstl * grow, nday=acc["nday"] + grow)
+

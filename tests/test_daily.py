@@ -54,7 +54,7 @@ def _unpack_day(a):
             for k, w in DY.DAY_FIELDS}
 
 
-def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1, seasonal=None):
+def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1, seasonal=None, stress=None):
     """One day begin (end=0) or day end (end=1) of the host build.  ``seasonal``: optional
     dict with the (3, n) state block, lat_rad, jday and a daily.PhenoParams; the block is
     updated in place."""
@@ -76,6 +76,8 @@ def host_day(end, f, day, td, sums, comp, prec, nsoilcall=1, seasonal=None):
         p.seasonal = seasonal["block"].ctypes.data
         p.lat_rad = seasonal["lat_rad"].ctypes.data
         p.jday = seasonal["jday"]
+    if stress:                       # dict with the (3, n) accumulator block and tmin (deg C)
+        p.stress, p.stress_accum_tmin = stress["block"].ctypes.data, float(stress["tmin"])
     ptr = lambda x: x.ctypes.data_as(C.c_void_p)
     lib = C.CDLL(HOSTCHECK)
     flags = abi.PREC_F64 if prec == "f64" else abi.PREC_MIXED
@@ -205,7 +207,7 @@ def test_host_day_begin_with_seasonal_states_matches_the_oracle(prec, jday):
 
 
 def test_day_params_struct_layout():
-    assert C.sizeof(DY.IbisDayParams) == 75 * 8 + 8 + 7 * 8 + 4 * 4 + 2 * 8
+    assert C.sizeof(DY.IbisDayParams) == 75 * 8 + 8 + 7 * 8 + 4 * 4 + 2 * 8 + 2 * 8
     p = DY.day_params(HSOI)
     assert p.cnr[4] == 100.0 and p.klm == 0.15 and p.ynb == 0.25 and p.k_scale == 1.0
     assert p.hsoi[6] == 4.0 and p.nsoilcall == 1 and p.ddays == 15.0
@@ -242,3 +244,33 @@ def test_product_precip_timing_matches_the_oracle():
         ref = ODR.precip_timing(precip, dtime, ul, us)
         for g, r in zip(got, ref):
             np.testing.assert_array_equal(g, r)
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_host_day_end_stress_accumulator(prec):
+    """Growing-season days add the canopies' stress factors and one day
+    (loops_gswp3.py:587-599): with tmin = -100 every day counts, so three day ends give
+    nday = 3 and sums of 3 x the factors; with tmin = 5 C only warmer cells move."""
+    n = 2000
+    f, day, td, sums, comp = day_case(n, seed=12)
+    dt = np.float64 if prec == "f64" else np.float32
+    su = np.asarray(f["soil.stresstu"], dt).astype(np.float64)
+    sl = np.asarray(f["soil.stresstl"], dt).astype(np.float64)
+    blk = np.zeros((3, n), dt)
+    for _ in range(3):
+        host_day(1, f, day, td, sums, comp, prec, stress=dict(block=blk, tmin=-100.0))
+    np.testing.assert_array_equal(blk[2], 3.0)
+    np.testing.assert_allclose(blk[0], 3.0 * su, rtol=1e-6 if prec == "mixed" else 1e-15)
+    np.testing.assert_allclose(blk[1], 3.0 * sl, rtol=1e-6 if prec == "mixed" else 1e-15)
+    start = np.stack([np.full(n, 0.5), np.full(n, 0.25), np.full(n, 7.0)]).astype(dt)
+    blk = start.copy()
+    host_day(1, f, day, td, sums, comp, prec, stress=dict(block=blk, tmin=5.0))
+    ref = OD.stress_accum(dict(zip(DY.STRESS_FIELDS, start.astype(np.float64))),
+                          np.asarray(td, dt).astype(np.float64), su, sl, 273.16, 5.0)
+    warm = np.asarray(td, dt) > dt(273.16 + 5.0)
+    assert 0 < warm.sum() < n
+    for j, k in enumerate(DY.STRESS_FIELDS):
+        np.testing.assert_allclose(blk[j], ref[k], rtol=1e-6 if prec == "mixed" else 1e-15,
+                                   err_msg=k)
+    np.testing.assert_array_equal(blk[:, ~warm], start[:, ~warm])
+

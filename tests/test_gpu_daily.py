@@ -191,3 +191,28 @@ def test_release_configuration_day_loop():
         err = np.abs(y - x).max() / max(np.abs(x).max(), 1e-30)
         print(f"\n{name}: mixed vs FP64 after 3 release-configuration days, {err:.2e}")
         assert err < 1e-3, name
+
+
+def test_stress_accumulator_through_the_day_loop():
+    """Three days with the stress accumulator attached (every day counts): nday = 3 and the
+    sums are the day-end stress factors summed, as loops_gswp3.py:587-599 accumulates them."""
+    n = 2048
+    f, slot, lat = synth.make_cells(n, 26)
+    st = IBISState.from_numpy(f, slot, synth.HSOI, root_fractions(), device="cuda",
+                              dtype=torch.float32)
+    day = DY.DayState(st, dict(plai=np.full((n, 12), 2.0), tc=np.full(n, 5.0),
+                               a10td=np.asarray(f["soil.tsoi"])[:, 0]))
+    w = DY.SyntheticWeather(lat, np.asarray(f["atm.psurf"]), seed=6, jday0=180)
+    p = DY.day_params(synth.HSOI)
+    blk = DY.attach_stress(st, p, tmin_c=-100.0)
+    seen = []
+    DY.run_days(st, day, w, DR.Accumulators(st), get_constants(), default_sim_params(), 3, p,
+                on_day=lambda d: seen.append((st.soil.stresstu[:n].clone(),
+                                              st.soil.stresstl[:n].clone())))
+    torch.cuda.synchronize()
+    assert torch.equal(blk[2, :n], torch.full((n,), 3.0, device="cuda"))
+    su = sum(s[0].double() for s in seen)
+    sl = sum(s[1].double() for s in seen)
+    torch.testing.assert_close(blk[0, :n].double(), su, rtol=1e-6, atol=1e-6)
+    torch.testing.assert_close(blk[1, :n].double(), sl, rtol=1e-6, atol=1e-6)
+    assert torch.equal(st.aux["stress.nday"], blk[2])
