@@ -66,6 +66,11 @@ NODE = {"soil": "SoilState", "snow": "SnowState", "canopy": "CanopyState", "atm"
 SUBSTATES = ("soil", "snow", "canopy", "atm", "vegetation", "carbon", "hydrology", "summary")
 #: SeasonalState's leaves in field order (python-port/ibis/state/seasonal.py:36-43)
 SEASONAL_FIELDS = ("acclim_x", "ssen_u", "ssen_l")
+#: the optional modules whose leading leaves map to the state's aux leaves: module name ->
+#: (aux prefix, leaves in field order); StressAccum's fourth leaf, u_smooth, is the annual
+#: driver's (python-port/ibis/state/stress_accum.py:15-21)
+AUX_MODULES = {"SeasonalState": ("seasonal", SEASONAL_FIELDS),
+               "StressAccum": ("stress", ("aystressu", "aystressl", "nday"))}
 #: leaf counts of the optional trailing chain, and of its first module in old files
 CHAIN_LEAVES = (4, 4, 3, 2)
 OLD_STRESS_LEAVES = 3
@@ -248,13 +253,16 @@ class Restart:
     def aux_fields(self) -> dict:
         """The optional per-cell leaves the step's switches read, where the file has them:
         the day's mean temperature ``atm.td`` and, in a file carrying the SeasonalState module
-        (state/seasonal.py:36-43), ``seasonal.acclim_x``, ``ssen_u``, ``ssen_l``."""
+        (state/seasonal.py:36-43), ``seasonal.acclim_x``, ``ssen_u``, ``ssen_l``; with the
+        StressAccum module (state/stress_accum.py:15-21), ``stress.aystressu``, ``aystressl``,
+        ``nday``."""
         out = {"atm.td": np.asarray(self.leaf("atm.td"), dtype=np.float64).reshape(self.ncell)}
-        o = self.module_offset.get("SeasonalState")
-        if o is not None:
-            for j, k in enumerate(SEASONAL_FIELDS):
-                out[f"seasonal.{k}"] = np.asarray(self.leaves[o + j],
-                                                  dtype=np.float64).reshape(self.ncell)
+        for mod, (pre, names) in AUX_MODULES.items():
+            o = self.module_offset.get(mod)
+            if o is not None:
+                for j, k in enumerate(names):
+                    out[f"{pre}.{k}"] = np.asarray(self.leaves[o + j],
+                                                   dtype=np.float64).reshape(self.ncell)
         return out
 
     def to_state(self, slot_type=None, *, device="cuda", dtype=torch.float32, pitch=None,
@@ -288,10 +296,11 @@ class Restart:
         i = self.index("vegetation.nytimes")
         old = np.asarray(self.leaves[i])
         self.leaves[i] = np.full(old.shape, state.nytimes, dtype=old.dtype)
-        o = self.module_offset.get("SeasonalState")
-        for j, k in enumerate(SEASONAL_FIELDS):
-            t = state.aux.get(f"seasonal.{k}")
-            if o is not None and t is not None:
-                old = self.leaves[o + j]
-                v = t[:state.n].detach().cpu().numpy()
-                self.leaves[o + j] = np.asarray(v).reshape(np.shape(old)).astype(old.dtype)
+        for mod, (pre, names) in AUX_MODULES.items():
+            o = self.module_offset.get(mod)
+            for j, k in enumerate(names):
+                t = state.aux.get(f"{pre}.{k}")
+                if o is not None and t is not None:
+                    old = self.leaves[o + j]
+                    v = t[:state.n].detach().cpu().numpy()
+                    self.leaves[o + j] = np.asarray(v).reshape(np.shape(old)).astype(old.dtype)

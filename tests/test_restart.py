@@ -159,7 +159,9 @@ def test_seasonal_states_map_to_the_state_and_back(tmp_path):
     nodes = ["None"] * (8 - len(mods)) + [
         f"CustomNode({m}[(Missing,)], [{', '.join(['*'] * c)}])" for m, c in mods]
     tree, r = _tree_with(nodes)
-    extra = [np.zeros(4, np.float32)] * 8 + [np.array([1.5, -2.0, 7.25, 0.0], np.float32),
+    stress = [np.array([0.5, 1.5, 2.0, 0.0], np.float32), np.array([0.25, 1.0, 2.0, 0.0], np.float32),
+              np.array([2.0, 3.0, 3.0, 0.0], np.float32), np.zeros(4, np.float32)]
+    extra = stress + [np.zeros(4, np.float32)] * 4 + [np.array([1.5, -2.0, 7.25, 0.0], np.float32),
                                              np.array([0.0, 12.0, 0.0, 3.0], np.float32),
                                              np.array([1.0, 0.0, 2.0, 0.0], np.float32)]
     rr = RS.Restart(r.leaves + extra, tree, r.meta)
@@ -176,3 +178,10 @@ def test_seasonal_states_map_to_the_state_and_back(tmp_path):
     np.testing.assert_array_equal(back.aux_fields()["seasonal.acclim_x"], [9.0, 8.0, 7.0, 6.0])
     assert back.leaves[-3].dtype == np.float32
     assert "seasonal.acclim_x" not in RS.Restart.load(FIX).aux_fields()
+    # the stress accumulator's first three leaves (state/stress_accum.py:15-21) likewise
+    np.testing.assert_array_equal(aux["stress.nday"], [2.0, 3.0, 3.0, 0.0])
+    np.testing.assert_array_equal(st.aux["stress.aystressl"].numpy(), [0.25, 1.0, 2.0, 0.0])
+    st.aux["stress.nday"][:4] = torch.tensor([4.0, 4.0, 4.0, 4.0])
+    rr.update(st)
+    np.testing.assert_array_equal(rr.aux_fields()["stress.nday"], [4.0] * 4)
+    np.testing.assert_array_equal(rr.leaves[len(r.leaves) + 3], np.zeros(4, np.float32))
